@@ -1,0 +1,171 @@
+"""Benchmark: batched GCDs/s on 100k-bit pairs vs one pair per launch.
+
+Workload (synthetic, seeded): NPAIRS pairs u = g a, v = g b with a 10k-bit
+shared factor g and 90k-bit cofactors (~100k-bit operands), 32-bit moduli.
+A step is one pass of the whole batch through the persistent kernel with
+the inputs already resident in HBM (mgcd_batch_run: launch, digits back to
+the host, conversion to positional limbs).  The one-pair-per-launch path
+(mgcd_gcd_limbs: staging, launch, read-back per pair) is timed on a sample
+for p50/p99 latency and its GCDs/s.  Results are checked bit-exact against
+GMP mpz_gcd on the CPU-baseline sample and by divisibility on every pair.
+
+Prints one JSON line (rank 0, N = 1 GPU only).
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__ as ge  # noqa: E402
+
+# MI355X peak VALU issue rate: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz lane-ops/s
+# (= the FP64 vector peak 78.6 TFLOP/s counting an FMA as two flops).
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
+
+# Algorithmic VALU lane-ops per unit of work ("Roofline").
+OPS_PER_RESIDUE_STEP = 6        # one Horner step per limb per modulus
+OPS_PER_ELIM_FIXED = 60         # 4 mulmod + reduce + key, per modulus-iteration
+OPS_PER_EEA_STEP = 14           # one quasi-quotient extended-Euclid step
+EEA_STEPS_32 = 0.843 * math.log(2**32) + 0.06  # mean Euclid steps, random inputs mod ~2^32
+OPS_PER_RECOVERY = 13           # 2 mulmod + reduce per modulus-digit
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pairs", type=int, default=4096)
+    ap.add_argument("--bits", type=int, default=100000)
+    ap.add_argument("--gbits", type=int, default=10000)
+    ap.add_argument("--L", type=int, default=32)
+    ap.add_argument("--per-launch-sample", type=int, default=32)
+    ap.add_argument("--workgroup", type=int, default=0)
+    ap.add_argument("--slots", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the modular GCD path is not sharded across GPUs "
+              "(see DESIGN.md, 'Sharding').", file=sys.stderr)
+        sys.exit(2)
+
+    m = ge.package()
+    w = ge.workload()
+    t0 = time.time()
+    pairs = w.headline(args.pairs, seed=7, nbits=args.bits, gbits=args.gbits)
+    log(f"generated {len(pairs)} pairs in {time.time() - t0:.1f}s")
+    ctx = m.Context(args.L)
+    batch = m.Batch.from_pairs(ctx, pairs, workgroup=args.workgroup, slots=args.slots)
+    T, K = batch.tiling()
+    log(f"batch resident; tiling T={T} K={K}")
+
+    for i in range(args.warmup):
+        batch.run()
+        log(f"warmup {i} done")
+    ctx.synchronize()
+    dev_times = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        batch.run()
+        dev_times.append(batch.timing()[0])
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = elapsed / args.steps * 1e3
+    value = args.pairs * args.steps / elapsed
+    log(f"batched: {ms_per_step:.1f} ms/step, {value:.1f} GCD/s, kernel {statistics.mean(dev_times)*1e3:.1f} ms")
+
+    # Correctness of every pair: g | u, g | v, g0 | g.
+    results = batch.results()
+    for i, (u, v) in enumerate(pairs):
+        g = results[i]
+        assert g > 0 and u % g == 0 and v % g == 0, f"pair {i}: result does not divide the operands"
+
+    # Work counts for the roofline.
+    res_steps = elim_units = rec_units = 0
+    for i in range(args.pairs):
+        nq, R, k = batch.pair_stats(i)
+        nu = (pairs[i][0].bit_length() + 31) // 32
+        nv = (pairs[i][1].bit_length() + 31) // 32
+        res_steps += nq * (nu + nv)
+        elim_units += R * nq - R * (R - 1) // 2 + nq  # initial keys + iterations
+        rec_units += sum(nq - R - j for j in range(k))
+    alg_ops = (res_steps * OPS_PER_RESIDUE_STEP + elim_units * (OPS_PER_ELIM_FIXED + EEA_STEPS_32 * OPS_PER_EEA_STEP)
+               + rec_units * OPS_PER_RECOVERY)
+    kern_s = statistics.mean(dev_times)
+    achieved = alg_ops / kern_s / 1e12
+
+    # One pair per launch.
+    lat = []
+    for i in range(args.per_launch_sample):
+        u, v = pairs[i]
+        t = time.perf_counter()
+        g = ctx.gcd(u, v)
+        lat.append(time.perf_counter() - t)
+        assert g == results[i], f"pair {i}: one-per-launch result differs from batched"
+    lat.sort()
+    p50 = statistics.median(lat)
+    p99 = lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]
+    single_rate = 1.0 / statistics.mean(lat)
+    log(f"one-per-launch: p50 {p50*1e3:.2f} ms p99 {p99*1e3:.2f} ms -> {single_rate:.1f} GCD/s")
+
+    cpu = None
+    exact_checked = 0
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        sample = min(args.pairs, 4096)
+        limbs, u_off, nu, v_off, nv = m.pack_pairs(pairs[:sample])
+        sec, gbits = O.time_mpz_gcd(limbs, u_off, nu, v_off, nv, threads)
+        for i in range(sample):
+            assert int(gbits[i]) == results[i].bit_length(), f"pair {i}: gcd size differs from mpz_gcd"
+        for i in range(min(sample, 64)):
+            assert results[i] == O.mpz_gcd(*pairs[i]), f"pair {i}: gcd differs from mpz_gcd"
+            exact_checked += 1
+        cpu = {"value": sample / sec, "unit": "GCD/s", "cores": threads, "kind": "reference",
+               "sample": f"GMP {O.gmp_version()} mpz_gcd (the reference's CPU path, testmodgcd.cpp:261-263) on the "
+                         f"first {sample} pairs of the same workload, {threads} threads pulling pairs from a queue; "
+                         f"{sec:.2f} s wall"}
+        log(f"cpu baseline: {cpu['value']:.1f} GCD/s on {threads} threads")
+
+    line = {
+        "metric": "GCDs/sec batched vs one-per-launch, 100k-bit pairs; p50/p99 latency; bit-exact",
+        "value": round(value, 3),
+        "unit": "GCD/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"batched {args.pairs} x {args.bits}-bit pairs, {args.gbits}-bit shared factor, "
+                               f"{args.L}-bit moduli", "pairs": args.pairs, "bits": args.bits,
+                   "gcd_bits": args.gbits, "L": args.L, "tiling": {"workgroup": T, "slots": K}},
+        "input_bits_per_s": round(value * 2 * args.bits, 1),
+        "one_per_launch": {"gcds_per_s": round(single_rate, 3), "p50_ms": round(p50 * 1e3, 3),
+                           "p99_ms": round(p99 * 1e3, 3), "sample": args.per_launch_sample},
+        "batched_vs_one_per_launch": round(value / single_rate, 2),
+        "bit_exact": {"mpz_gcd_equal": exact_checked, "divisibility_checked": args.pairs},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_LANE_OPS / 1e12, 2),
+                     "unit": "Tlane-op/s", "frac": round(achieved / (VALU_PEAK_LANE_OPS / 1e12), 4),
+                     "traffic": None, "kernel_ms": round(kern_s * 1e3, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

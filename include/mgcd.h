@@ -1,0 +1,138 @@
+/* mgcd.h -- C ABI of the MI355X modular integer GCD library (libmgcd.so).
+ *
+ * This is the thin, GMP-free boundary that the drop-in C++ class
+ * GmpCuda::GmpCudaDevice (GmpCuda.h) calls.  Integers cross it as arrays of
+ * 32-bit limbs, least significant limb first (the layout mpz_export(order=-1,
+ * size=4, endian=0) produces, GmpCudaDevice-gcd.cu:71-73).  No C++ exception
+ * and no HIP type crosses it; every entry point returns an mgcd_status.
+ *
+ * Threading: a context may be used by one host thread at a time.  Batches
+ * and servers belong to the context that created them.
+ */
+#ifndef MGCD_H
+#define MGCD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The first six map one-to-one to the std::runtime_error
+ * messages of the reference (GmpCudaDevice.cu:101, GmpCudaDevice-gcd.cu:78,
+ * :82, :107, :108, :109); mgcd_strerror returns those exact strings. */
+typedef enum {
+  MGCD_OK = 0,
+  MGCD_ERR_INVALID_DEVICE = 1,     /* "Invalid device number." */
+  MGCD_ERR_NOT_ENOUGH_MODULI = 2,  /* "Not enough moduli available for given input." */
+  MGCD_ERR_TOO_MANY_THREADS = 3,   /* "Cannot allocate enough threads to support computation." */
+  MGCD_ERR_REDUX = 4,              /* "Ran out of moduli in the reduction loop." */
+  MGCD_ERR_RECOVERY = 5,           /* "Ran out of moduli in the recovery loop." */
+  MGCD_ERR_KERNEL = 6,             /* "Unknown error in the gcd kernel." */
+  MGCD_ERR_HIP = 7,                /* a HIP runtime call failed */
+  MGCD_ERR_ARG = 8,                /* invalid argument */
+  MGCD_ERR_TABLE = 9,              /* moduli table file missing or invalid */
+  MGCD_ERR_CAPACITY = 10,          /* caller's output buffer too small */
+  MGCD_ERR_NOMEM = 11,             /* host or device allocation failed */
+  MGCD_ERR_TIMEOUT = 12,           /* a bounded in-kernel wait expired */
+  MGCD_ERR_BUSY = 13               /* server: admission budget exhausted */
+} mgcd_status;
+
+/* One signed mixed-radix digit: the pair_t the reference kernel hands back
+ * to the host (GmpCuda.h:112).  Digit i of a result G satisfies
+ * G = g_1 + m_1 (g_2 + m_2 (g_3 + ...)), g_i in [-(m_i+1)/2, (m_i-3)/2]. */
+typedef struct {
+  uint32_t modulus;
+  int32_t value;
+} mgcd_pair_t;
+
+typedef struct mgcd_ctx mgcd_ctx;
+typedef struct mgcd_batch mgcd_batch;
+typedef struct mgcd_server mgcd_server;
+
+/* Human-readable message for a status. */
+const char* mgcd_strerror(int status);
+/* Detail of the last failure on the calling thread (HIP call, table path). */
+const char* mgcd_last_error_detail(void);
+
+/* Device selection (replaces GmpCudaDevice::setDevice, GmpCudaDevice.cu:96-103):
+ * MGCD_ERR_INVALID_DEVICE if dev is out of range. */
+int mgcd_device_count(int* count);
+int mgcd_set_device(int dev);
+
+/* Number of moduli the algorithm is started with for an input whose larger
+ * operand has `bits` bits, at modulus width L (numModuliNeededFor,
+ * GmpCudaDevice-gcd.cu:49-55): roundUp(ceil(C_L*bits/ln(bits)), 256), C_L =
+ * 1.6 - 0.015 L, evaluated in float as the reference does. */
+int mgcd_num_moduli_needed(uint64_t bits, int L);
+
+/* Context: a device, a moduli table of width L in {22, 27, 32} uploaded to
+ * it, streams and pooled device buffers (replaces the GmpCudaDevice
+ * constructor, GmpCudaDevice.cu:107-138).  table_path may be NULL: the
+ * library then loads moduli_L<L>.bin from the directory named by
+ * $MGCD_TABLE_DIR, else from ../data next to libmgcd.so. */
+int mgcd_ctx_create(mgcd_ctx** out, int device, int L, const char* table_path);
+int mgcd_ctx_destroy(mgcd_ctx* ctx);
+int mgcd_ctx_L(const mgcd_ctx* ctx);
+int mgcd_ctx_num_moduli(const mgcd_ctx* ctx);           /* table size */
+int mgcd_ctx_max_grid(const mgcd_ctx* ctx);             /* getMaxGridSize (GmpCuda.h:118) */
+int mgcd_ctx_moduli(const mgcd_ctx* ctx, uint32_t* out, size_t cap); /* copy of the table */
+/* Blocks until all work queued on the context's device has finished. */
+int mgcd_ctx_synchronize(mgcd_ctx* ctx);
+/* Seconds the last gcd kernel launch of this context took (HIP events). */
+double mgcd_ctx_last_kernel_seconds(const mgcd_ctx* ctx);
+
+/* One pair, one launch sequence: the path of GmpCudaDevice::gcd
+ * (GmpCudaDevice-gcd.cu:57-132) without GMP.  u, v may come in either order
+ * and may be zero.  The gcd is written to g (cap limbs), its limb count to
+ * *ng (0 for a zero gcd).  Blocking. */
+int mgcd_gcd_limbs(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv,
+                   uint32_t* g, size_t cap, size_t* ng);
+
+/* As mgcd_gcd_limbs, and also returns the kernel's signed mixed-radix
+ * digits (least significant first) and the elimination pivots (p, b) in
+ * the order the elimination loop used them.  Any of the extra buffers may
+ * be NULL with capacity 0; *nd / *nt always receive the full counts.  The
+ * operands are taken in the order given (u must have at least as many bits
+ * as v) so the kernel sees exactly what the reference kernel would. */
+int mgcd_gcd_limbs_trace(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv,
+                         uint32_t* g, size_t cap, size_t* ng,
+                         mgcd_pair_t* digits, size_t dcap, size_t* nd,
+                         mgcd_pair_t* pivots, size_t tcap, size_t* nt);
+
+/* Batch of pairs for the persistent kernel.  Limbs of all operands are
+ * packed in `limbs`; pair i is (limbs + u_off[i], nu[i]) and
+ * (limbs + v_off[i], nv[i]).  The inputs are copied to device memory once,
+ * at creation; mgcd_batch_run computes every gcd from the resident copy.
+ * opts_workgroup / opts_slots: 0 lets the autotuner choose the tiling. */
+int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, const uint32_t* nu,
+                      const uint64_t* v_off, const uint32_t* nv, size_t npairs,
+                      int opts_workgroup, int opts_slots, mgcd_batch** out);
+int mgcd_batch_run(mgcd_batch* b);
+int mgcd_batch_status(const mgcd_batch* b, size_t i);
+int mgcd_batch_result(const mgcd_batch* b, size_t i, uint32_t* g, size_t cap, size_t* ng);
+/* Seconds the last run spent in its device launch sequence (HIP events on
+ * the context's stream), and in host-side digit conversion. */
+int mgcd_batch_timing(const mgcd_batch* b, double* device_s, double* host_s);
+/* Work done for pair i in the last run: N_Q, elimination iterations R and
+ * recovery digits k (0 for pairs answered on the host). */
+int mgcd_batch_pair_stats(const mgcd_batch* b, size_t i, uint32_t* nq, uint32_t* iters, uint32_t* digits);
+/* The tiling chosen: threads per workgroup, moduli slots per thread. */
+int mgcd_batch_tiling(const mgcd_batch* b, int* workgroup, int* slots);
+int mgcd_batch_destroy(mgcd_batch* b);
+
+/* Exhaustive certification of the FP32-reciprocal quasi-quotient on this
+ * device (the role of certifyQuoRemQuasi.cu):
+ *   mode 0: every x in [1, 2^23), y in [1, 2^22)  (the float domain);
+ *   mode 1: for width L, every y in [1, 2^L) and every quotient k, the two
+ *           endpoints x = k*y and x = (k+1)*y - 1 (the estimate is monotone
+ *           in x, so the endpoints decide every x in between).
+ * *failures receives the number of (x, y) violating the postcondition. */
+int mgcd_certify_quasi_quotient(mgcd_ctx* ctx, int mode, int L, uint64_t* failures, uint64_t* checked);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MGCD_H */

@@ -1,0 +1,127 @@
+// mgcd_arith.h -- exact modular arithmetic for one modulus per lane, gfx950.
+//
+// Residues are held as doubles in a "signed reduced" form: an integer r with
+// r == x (mod m) and |r| <= m/2 + 2^13.  Every value fits a double exactly
+// (m < 2^32), and the full-rate FP64 FMA of CDNA4 does the reductions:
+//
+//   mulmod(a, b): h = fl(a*b), l = fma(a, b, -h) (exact low part),
+//                 q = rint(h * minv), r = fma(-q, m, h) + l.
+//     For |a|, |b| <= m + 2^14 the product is < 2^64.5, so h differs from
+//     a*b by at most 2^11, |h*minv - a*b/m| < 2^-19, and |a*b - q*m| <=
+//     m/2 + 2^13.  fma(-q, m, h) is then an integer below 2^53 and exact,
+//     and adding l is exact.
+//   reduce(x):    q = rint(x * minv), r = fma(-q, m, x)  (|x| < 2^52).
+//   canon(x):     x + m if x < 0 else x  -> [0, m).
+//
+// The only inverse computed is modinv (Knuth's Algorithm X with remainders
+// and cofactors kept exactly, modInv.h:141-188), whose quotients come from
+// the FP32 reciprocal v_rcp_f32 as quasi-quotients:
+//
+//   q~(x, y) = floor( fl32(x) * (rcp(fl32(y)) * QBIAS) ),  QBIAS = 1 - 2^-20
+//
+// The bias makes q~ never exceed floor(x/y) (the combined relative error of
+// the conversions, v_rcp_f32 and two roundings is below 2^-21), so every
+// remainder x - q~ y is non-negative; it is computed exactly with an FMA
+// (the exact integer correction).  When q~ falls short the remainder lands
+// in [y, 2y) and the next step has quotient 0, which only swaps the pair.
+// The "never exceeds" postcondition, and "short by at most one when
+// floor(x/y) < 2^19", are certified exhaustively on the device by
+// mgcd_certify_quasi_quotient (float domain x < 2^23, y < 2^22 point by
+// point, and all widths up to 32 bits through the monotone endpoints).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mgcd {
+namespace dev {
+
+constexpr float QBIAS = 1.0f - 0x1p-20f;
+
+struct ModD {
+  double m;     // modulus
+  double minv;  // fl(1/m)
+};
+
+__device__ __forceinline__ double reduce(double x, ModD M) {
+  const double q = __builtin_rint(x * M.minv);
+  return __builtin_fma(-q, M.m, x);
+}
+
+__device__ __forceinline__ double canon(double x, ModD M) { return x < 0.0 ? x + M.m : x; }
+
+// Signed reduced -> the reference's toSigned range [-(m+1)/2, (m-3)/2]
+// (getGcdKernel.cu:278-281).
+__device__ __forceinline__ int32_t to_signed(double r, ModD M) {
+  const double x = canon(r, M);
+  const double half = __builtin_fma(M.m, 0.5, -0.5);  // (m-1)/2 exactly
+  return (int32_t)(x >= half ? x - M.m : x);
+}
+
+// Quasi-quotient from the FP32 reciprocal (see file comment).
+__device__ __forceinline__ float quasi_quo_f(float xf, float yf) {
+  const float c = __builtin_amdgcn_rcpf(yf) * QBIAS;
+  return __builtin_floorf(xf * c);
+}
+
+// Moduli up to 32 bits: products need the two-product low part.
+struct Arith32 {
+  static __device__ __forceinline__ double mulmod(double a, double b, ModD M) {
+    const double h = a * b;
+    const double l = __builtin_fma(a, b, -h);
+    const double q = __builtin_rint(h * M.minv);
+    return __builtin_fma(-q, M.m, h) + l;
+  }
+  // y in [1, m) canonical; returns y^-1 mod m in [0, m).
+  // Invariant r_i == s_i * y (mod m); stops at r1 <= 1.
+  static __device__ double modinv(double y, ModD M) {
+    double r0 = M.m, r1 = y, s0 = 0.0, s1 = 1.0;
+    float f0 = (float)M.m, f1 = (float)y;
+    while (r1 > 1.0) {
+      const double q = (double)quasi_quo_f(f0, f1);
+      const double r2 = __builtin_fma(-q, r1, r0);
+      const double s2 = __builtin_fma(-q, s1, s0);
+      r0 = r1; f0 = f1; s0 = s1;
+      r1 = r2; s1 = s2; f1 = (float)r2;
+    }
+    const double s = (r1 == 1.0) ? s1 : s0;
+    return s < 0.0 ? s + M.m : s;
+  }
+};
+
+// Moduli below 2^22: residues, products (< 2^44) and the whole extended
+// Euclid fit FP32/FP64 exactly, so the inverse runs in FP32 only.
+struct Arith22 {
+  static __device__ __forceinline__ double mulmod(double a, double b, ModD M) {
+    const double h = a * b;  // exact: |a*b| < 2^46
+    const double q = __builtin_rint(h * M.minv);
+    return __builtin_fma(-q, M.m, h);
+  }
+  static __device__ double modinv(double y, ModD M) {
+    const float m = (float)M.m;
+    float r0 = m, r1 = (float)y, s0 = 0.0f, s1 = 1.0f;
+    while (r1 > 1.0f) {
+      const float q = quasi_quo_f(r0, r1);
+      const float r2 = __builtin_fmaf(-q, r1, r0);
+      const float s2 = __builtin_fmaf(-q, s1, s0);
+      r0 = r1; s0 = s1;
+      r1 = r2; s1 = s2;
+    }
+    const float s = (r1 == 1.0f) ? s1 : s0;
+    return (double)(s < 0.0f ? s + m : s);
+  }
+};
+
+// One Horner step of the residue reduction: (r * 2^32 + limb) mod m, signed
+// reduced in and out (modMP, getGcdKernel.cu:297-324).  t = r*2^32 is exact;
+// x = t + limb is rounded but only feeds the quotient estimate; t - q*m is
+// an integer below 2^34 and exact in the FMA.
+__device__ __forceinline__ double horner_step(double r, double limb, ModD M) {
+  const double t = r * 4294967296.0;
+  const double x = t + limb;
+  const double q = __builtin_rint(x * M.minv);
+  return __builtin_fma(-q, M.m, t) + limb;
+}
+
+}  // namespace dev
+}  // namespace mgcd

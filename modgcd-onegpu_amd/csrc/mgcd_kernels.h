@@ -1,0 +1,80 @@
+// mgcd_kernels.h -- host-visible declarations shared by the kernels
+// (mgcd_kernels.hip) and the runtime (mgcd_runtime.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/mgcd.h"
+
+namespace mgcd {
+
+// Per-pair descriptor in device memory.  The larger operand is u.
+struct PairDesc {
+  uint64_t u_off;      // limb offset of u
+  uint64_t v_off;      // limb offset of v
+  uint32_t nu, nv;     // limb counts (no leading zero limbs)
+  uint32_t nq;         // N_Q: moduli the pair starts with
+  uint32_t G;          // workgroups the pair is split over
+  uint64_t dig_off;    // first digit slot in the digit buffer
+  uint64_t bar_off;    // first barrier slot (2*G uint64 slots), if G > 1
+  uint64_t trace_off;  // first pivot-trace slot
+  uint32_t trace_cap;  // 0: no trace
+  uint32_t pad;
+};
+
+// One unit of the persistent work queue: part `part` of pair `pair`.
+struct WorkItem {
+  uint32_t pair;
+  uint32_t part;
+};
+
+// Per-pair result header.
+struct PairOut {
+  int32_t status;   // KST_*
+  uint32_t ndig;    // digits written
+  uint32_t ntrace;  // elimination iterations (pivots used)
+  uint32_t pad;
+};
+
+enum : int32_t { KST_OK = 0, KST_REDUX = 1, KST_RECOVERY = 2, KST_TIMEOUT = 3, KST_NOTRUN = 4 };
+
+struct KernelArgs {
+  const uint32_t* limbs;
+  const PairDesc* pairs;
+  const WorkItem* items;
+  uint32_t nitems;
+  int L;
+  uint32_t* queue;             // [0]: next work item
+  const uint32_t* mod_u32;     // moduli table
+  const double* mod_d;         // moduli as double
+  const double* mod_inv;       // fl(1 / m)
+  unsigned long long* bar;     // barrier slots (zeroed before each launch)
+  mgcd_pair_t* digits;
+  mgcd_pair_t* trace;
+  PairOut* out;
+  uint32_t spin_limit;         // polls before a barrier wait gives up
+};
+
+// Kernel variants: threads per workgroup T and moduli slots per thread K,
+// for the two arithmetic classes (L <= 22: FP32 inverse; L <= 32).
+struct KernelVariant {
+  int T, K, wide;  // wide = 1 for the 32-bit class
+  const void* fn;  // __global__ function pointer for hipLaunchCooperativeKernel
+};
+
+int kernel_variant_count();
+const KernelVariant& kernel_variant(int i);
+// Finds the variant with exactly (T, K, wide), or -1.
+int find_kernel_variant(int T, int K, int wide);
+
+// Small helper kernels.
+hipError_t launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+
+// Certifier launches (see mgcd_certify_quasi_quotient).
+hipError_t launch_certify_float(unsigned long long* fail, unsigned long long* checked, int grid, hipStream_t s);
+hipError_t launch_certify_endpoints(int L, uint64_t y_lo, uint64_t y_hi, unsigned long long* fail,
+                                    unsigned long long* checked, int grid, hipStream_t s);
+
+}  // namespace mgcd

@@ -1,0 +1,785 @@
+// mgcd_runtime.cpp -- host runtime behind the C ABI of include/mgcd.h.
+//
+// Owns the device context (moduli table in HBM, stream, pooled buffers,
+// occupancy of each kernel variant), plans launches (how many workgroups a
+// pair is split over, which (T, K) variant runs a batch), launches the
+// persistent kernel cooperatively so that all of its workgroups are
+// resident (the cross-workgroup waits need that), and converts the signed
+// mixed-radix digits back to positional limbs (the host Horner of
+// GmpCudaDevice-gcd.cu:243-256, without GMP).
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "mgcd_kernels.h"
+#include "mgcd_table.h"
+
+using namespace mgcd;
+
+namespace {
+
+constexpr uint32_t kMaxModuli = (1u << 19) - 1;  // kernel key field width
+constexpr uint32_t kSpinLimit = 1u << 24;        // polls before a wait reports a timeout
+
+const char* kMessages[] = {
+    "OK",
+    "Invalid device number.",
+    "Not enough moduli available for given input.",
+    "Cannot allocate enough threads to support computation.",
+    "Ran out of moduli in the reduction loop.",
+    "Ran out of moduli in the recovery loop.",
+    "Unknown error in the gcd kernel.",
+    "HIP runtime error.",
+    "Invalid argument.",
+    "Moduli table missing or invalid.",
+    "Output buffer too small.",
+    "Out of memory.",
+    "A device-side wait timed out.",
+    "Admission budget exhausted.",
+};
+
+thread_local std::string g_last_error;
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      g_last_error = std::string(#expr) + ": " + hipGetErrorString(e_);              \
+      return MGCD_ERR_HIP;                                                            \
+    }                                                                                 \
+  } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+size_t trim(const uint32_t* x, size_t n) {
+  while (n > 0 && x[n - 1] == 0) --n;
+  return n;
+}
+uint64_t bit_length(const uint32_t* x, size_t n) {
+  n = trim(x, n);
+  if (n == 0) return 0;
+  return 32 * (uint64_t)(n - 1) + (32 - __builtin_clz(x[n - 1]));
+}
+int cmp_mag(const uint32_t* a, size_t na, const uint32_t* b, size_t nb) {
+  na = trim(a, na);
+  nb = trim(b, nb);
+  if (na != nb) return na < nb ? -1 : 1;
+  for (size_t i = na; i-- > 0;)
+    if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+  return 0;
+}
+
+// Signed mixed-radix digits (least significant first) -> |G| as limbs:
+// G = g_k; G = G * m_i + g_i for i = k-1 .. 1 (GmpCudaDevice-gcd.cu:243-256).
+// When G != 0, |G * m_i| >= m_i > |g_i|, so the sign never flips mid-way.
+void digits_to_limbs(const mgcd_pair_t* d, size_t k, std::vector<uint32_t>& mag) {
+  mag.clear();
+  if (k == 0) return;
+  bool neg = d[k - 1].value < 0;
+  uint32_t top = neg ? 0u - (uint32_t)d[k - 1].value : (uint32_t)d[k - 1].value;
+  if (top) mag.push_back(top);
+  for (size_t i = k - 1; i-- > 0;) {
+    const uint64_t p = d[i].modulus;
+    uint64_t carry = 0;
+    for (uint32_t& w : mag) {
+      const uint64_t x = (uint64_t)w * p + carry;
+      w = (uint32_t)x;
+      carry = x >> 32;
+    }
+    if (carry) mag.push_back((uint32_t)carry);
+    const int32_t g = d[i].value;
+    if (g == 0) continue;
+    const bool gneg = g < 0;
+    const uint32_t ga = gneg ? 0u - (uint32_t)g : (uint32_t)g;
+    if (mag.empty()) {
+      mag.push_back(ga);
+      neg = gneg;
+    } else if (gneg == neg) {
+      uint64_t c = ga;
+      for (size_t j = 0; c && j < mag.size(); ++j) {
+        const uint64_t x = (uint64_t)mag[j] + c;
+        mag[j] = (uint32_t)x;
+        c = x >> 32;
+      }
+      if (c) mag.push_back((uint32_t)c);
+    } else {
+      uint64_t b = ga;  // mag > ga
+      for (size_t j = 0; b && j < mag.size(); ++j) {
+        const uint64_t x = (uint64_t)mag[j] - b;
+        mag[j] = (uint32_t)x;
+        b = (x >> 63) & 1;
+      }
+      while (!mag.empty() && mag.back() == 0) mag.pop_back();
+    }
+  }
+}
+
+std::string library_dir() {
+  Dl_info info;
+  if (dladdr(reinterpret_cast<void*>(&library_dir), &info) && info.dli_fname) {
+    std::string p(info.dli_fname);
+    size_t s = p.rfind('/');
+    if (s != std::string::npos) return p.substr(0, s);
+  }
+  return ".";
+}
+
+int kst_to_status(int32_t k) {
+  switch (k) {
+    case KST_OK: return MGCD_OK;
+    case KST_REDUX: return MGCD_ERR_REDUX;
+    case KST_RECOVERY: return MGCD_ERR_RECOVERY;
+    case KST_TIMEOUT: return MGCD_ERR_TIMEOUT;
+    default: return MGCD_ERR_KERNEL;
+  }
+}
+
+}  // namespace
+
+struct mgcd_ctx {
+  int device = 0;
+  int L = 32;
+  int wide = 1;
+  int num_cus = 0;
+  std::vector<uint32_t> moduli;
+  DevBuf d_mod_u32, d_mod_d, d_mod_inv;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<int> max_resident;  // per kernel variant: co-resident workgroups
+  // pooled buffers of the single-pair path
+  DevBuf limbs, pairs, items, queue, bar, digits, trace, out;
+  double last_kernel_s = 0.0;
+};
+
+// A prepared launch over device-resident pairs.
+struct LaunchSet {
+  int variant = -1;
+  uint32_t nitems = 0;
+  uint32_t grid = 0;
+  size_t nbar = 0;       // barrier slots
+  size_t ndigits = 0;    // digit slots
+  size_t ntrace = 0;     // trace slots
+};
+
+namespace {
+
+// Splits pairs over workgroups of variant `vi`; fills descriptors' G,
+// offsets, and the work-item list (pairs in decreasing order of work so
+// multi-part pairs are popped first).  Returns MGCD_OK or an error.
+int plan_items(mgcd_ctx* ctx, int vi, std::vector<PairDesc>& pd, std::vector<WorkItem>& items, LaunchSet& ls,
+               uint32_t trace_cap) {
+  const KernelVariant& kv = kernel_variant(vi);
+  const uint32_t per_wg = (uint32_t)(kv.T * kv.K);
+  const uint32_t maxres = (uint32_t)ctx->max_resident[vi];
+  if (maxres == 0) return MGCD_ERR_TOO_MANY_THREADS;
+  size_t bar = 0, dig = 0, tr = 0;
+  uint32_t maxG = 1;
+  for (auto& p : pd) {
+    p.G = (p.nq + per_wg - 1) / per_wg;
+    if (p.G > maxres) return MGCD_ERR_TOO_MANY_THREADS;
+    maxG = std::max(maxG, p.G);
+    p.bar_off = bar;
+    if (p.G > 1) bar += 2 * (size_t)p.G;
+    p.dig_off = dig;
+    dig += p.nq;
+    p.trace_off = tr;
+    p.trace_cap = trace_cap;
+    tr += trace_cap;
+  }
+  std::vector<uint32_t> order(pd.size());
+  for (uint32_t i = 0; i < pd.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    const uint64_t wx = (uint64_t)pd[x].nq * pd[x].nq, wy = (uint64_t)pd[y].nq * pd[y].nq;
+    return wx > wy;
+  });
+  items.clear();
+  for (uint32_t i : order)
+    for (uint32_t g = 0; g < pd[i].G; ++g) items.push_back(WorkItem{i, g});
+  ls.variant = vi;
+  ls.nitems = (uint32_t)items.size();
+  ls.grid = std::min<uint32_t>(ls.nitems, maxres);
+  if (ls.grid < maxG) return MGCD_ERR_TOO_MANY_THREADS;
+  ls.nbar = bar;
+  ls.ndigits = dig;
+  ls.ntrace = tr;
+  return MGCD_OK;
+}
+
+// Clears the queue counter and barrier slots, then launches the persistent
+// kernel cooperatively on ctx->stream; records events around the kernel.
+int launch(mgcd_ctx* ctx, const LaunchSet& ls, const KernelArgs& ka) {
+  const KernelVariant& kv = kernel_variant(ls.variant);
+  HIP_TRY(launch_fill_u32(ka.queue, 1, 0u, ctx->stream));
+  if (ls.nbar) HIP_TRY(launch_fill_u64(ka.bar, ls.nbar, 0ull, ctx->stream));
+  KernelArgs args = ka;
+  void* params[] = {&args};
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  HIP_TRY(hipLaunchCooperativeKernel(kv.fn, dim3(ls.grid), dim3(kv.T), params, 0, ctx->stream));
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  return MGCD_OK;
+}
+
+int choose_single_variant(mgcd_ctx* ctx, uint32_t nq) {
+  const int w = ctx->wide;
+  if (nq <= 1024) return find_kernel_variant(1024, 1, w);
+  for (int K : {1, 2, 4, 8, 16}) {
+    const int vi = find_kernel_variant(256, K, w);
+    if (vi < 0) continue;
+    const uint32_t G = (nq + 256 * K - 1) / (256 * K);
+    if (G <= (uint32_t)ctx->max_resident[vi]) return vi;
+  }
+  return -1;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* mgcd_strerror(int status) {
+  if (status < 0 || status >= (int)(sizeof(kMessages) / sizeof(kMessages[0]))) return "Unknown status.";
+  return kMessages[status];
+}
+
+int mgcd_device_count(int* count) {
+  if (!count) return MGCD_ERR_ARG;
+  HIP_TRY(hipGetDeviceCount(count));
+  return MGCD_OK;
+}
+
+int mgcd_set_device(int dev) {
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (dev < 0 || dev >= n) return MGCD_ERR_INVALID_DEVICE;
+  HIP_TRY(hipSetDevice(dev));
+  return MGCD_OK;
+}
+
+int mgcd_num_moduli_needed(uint64_t bits, int L) {
+  // GmpCudaDevice-gcd.cu:49-55, in float as the reference computes it.
+  const float C_L = (float)(1.6 - 0.015 * L);
+  float x = C_L * (float)bits;
+  x = x / logf((float)bits);
+  const int n = (int)ceilf(x);
+  return ((n - 1) / 256 + 1) * 256;
+}
+
+int mgcd_ctx_create(mgcd_ctx** out, int device, int L, const char* table_path) {
+  if (!out || (L != 22 && L != 27 && L != 32)) return MGCD_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return MGCD_ERR_INVALID_DEVICE;
+  HIP_TRY(hipSetDevice(device));
+
+  std::string path;
+  if (table_path && *table_path) {
+    path = table_path;
+  } else {
+    const char* dir = std::getenv("MGCD_TABLE_DIR");
+    std::string d = dir && *dir ? std::string(dir) : library_dir() + "/../data";
+    path = d + "/moduli_L" + std::to_string(L) + ".bin";
+  }
+  int fileL = 0;
+  std::vector<uint32_t> mod;
+  const std::string err = readTable(path, &fileL, &mod);
+  if (!err.empty()) {
+    g_last_error = err;
+    return MGCD_ERR_TABLE;
+  }
+  if (fileL != L) {
+    g_last_error = "table width mismatch in " + path;
+    return MGCD_ERR_TABLE;
+  }
+  if (mod.size() > kMaxModuli) mod.resize(kMaxModuli);
+
+  mgcd_ctx* ctx = new mgcd_ctx();
+  ctx->device = device;
+  ctx->L = L;
+  ctx->wide = L > 22 ? 1 : 0;
+  ctx->moduli = std::move(mod);
+  hipDeviceProp_t props;
+  hipError_t e = hipGetDeviceProperties(&props, device);
+  if (e != hipSuccess || props.warpSize != 64) {
+    delete ctx;
+    g_last_error = "device properties unavailable or wavefront size != 64";
+    return MGCD_ERR_HIP;
+  }
+  ctx->num_cus = props.multiProcessorCount;
+  const size_t nm = ctx->moduli.size();
+  std::vector<double> md(nm), mi(nm);
+  for (size_t i = 0; i < nm; ++i) {
+    md[i] = (double)ctx->moduli[i];
+    mi[i] = 1.0 / md[i];
+  }
+  int rc = MGCD_OK;
+  auto fail = [&](int code) {
+    rc = code;
+    return code;
+  };
+  if (ctx->d_mod_u32.ensure(nm * 4) != hipSuccess || ctx->d_mod_d.ensure(nm * 8) != hipSuccess ||
+      ctx->d_mod_inv.ensure(nm * 8) != hipSuccess) {
+    fail(MGCD_ERR_NOMEM);
+  } else if (hipMemcpy(ctx->d_mod_u32.p, ctx->moduli.data(), nm * 4, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(ctx->d_mod_d.p, md.data(), nm * 8, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(ctx->d_mod_inv.p, mi.data(), nm * 8, hipMemcpyHostToDevice) != hipSuccess ||
+             hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+             hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    fail(MGCD_ERR_HIP);
+  }
+  if (rc == MGCD_OK) {
+    ctx->max_resident.assign(kernel_variant_count(), 0);
+    for (int i = 0; i < kernel_variant_count(); ++i) {
+      const KernelVariant& kv = kernel_variant(i);
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kv.fn, kv.T, 0) != hipSuccess) per_cu = 0;
+      ctx->max_resident[i] = per_cu * ctx->num_cus;
+    }
+  }
+  if (rc != MGCD_OK) {
+    mgcd_ctx_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return MGCD_OK;
+}
+
+int mgcd_ctx_destroy(mgcd_ctx* ctx) {
+  if (!ctx) return MGCD_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (DevBuf* b : {&ctx->d_mod_u32, &ctx->d_mod_d, &ctx->d_mod_inv, &ctx->limbs, &ctx->pairs, &ctx->items,
+                    &ctx->queue, &ctx->bar, &ctx->digits, &ctx->trace, &ctx->out})
+    b->release();
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MGCD_OK;
+}
+
+int mgcd_ctx_L(const mgcd_ctx* ctx) { return ctx ? ctx->L : 0; }
+int mgcd_ctx_num_moduli(const mgcd_ctx* ctx) { return ctx ? (int)ctx->moduli.size() : 0; }
+int mgcd_ctx_max_grid(const mgcd_ctx* ctx) {
+  if (!ctx) return 0;
+  const int vi = find_kernel_variant(256, 1, ctx->wide);
+  return vi < 0 ? 0 : ctx->max_resident[vi];
+}
+int mgcd_ctx_moduli(const mgcd_ctx* ctx, uint32_t* out, size_t cap) {
+  if (!ctx || !out || cap < ctx->moduli.size()) return MGCD_ERR_CAPACITY;
+  std::memcpy(out, ctx->moduli.data(), ctx->moduli.size() * 4);
+  return MGCD_OK;
+}
+
+const char* mgcd_last_error_detail(void) { return g_last_error.c_str(); }
+
+int mgcd_gcd_limbs_trace(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint32_t* g,
+                         size_t cap, size_t* ng, mgcd_pair_t* digits, size_t dcap, size_t* nd,
+                         mgcd_pair_t* pivots, size_t tcap, size_t* nt) {
+  if (!ctx || (nu && !u) || (nv && !v) || !ng) return MGCD_ERR_ARG;
+  HIP_TRY(hipSetDevice(ctx->device));
+  nu = trim(u, nu);
+  nv = trim(v, nv);
+  if (nd) *nd = 0;
+  if (nt) *nt = 0;
+  *ng = 0;
+  const uint64_t bits = bit_length(u, nu);
+  if (bits == 0) return MGCD_ERR_ARG;  // the reference's sizing divides by ln(1)
+  const int nqi = mgcd_num_moduli_needed(bits, ctx->L);
+  if (nqi <= 0 || (size_t)nqi > ctx->moduli.size()) return MGCD_ERR_NOT_ENOUGH_MODULI;
+  const uint32_t nq = (uint32_t)nqi;
+  const int vi = choose_single_variant(ctx, nq);
+  if (vi < 0) return MGCD_ERR_TOO_MANY_THREADS;
+
+  std::vector<PairDesc> pd(1);
+  std::memset(pd.data(), 0, sizeof(PairDesc));
+  pd[0].u_off = 0;
+  pd[0].v_off = nu;
+  pd[0].nu = (uint32_t)nu;
+  pd[0].nv = (uint32_t)nv;
+  pd[0].nq = nq;
+  std::vector<WorkItem> items;
+  LaunchSet ls;
+  const uint32_t trace_cap = (pivots && tcap) ? (uint32_t)std::min<size_t>(tcap, nq) : 0;
+  int rc = plan_items(ctx, vi, pd, items, ls, trace_cap);
+  if (rc != MGCD_OK) return rc;
+
+  if (ctx->limbs.ensure((nu + nv + 1) * 4) != hipSuccess || ctx->pairs.ensure(sizeof(PairDesc)) != hipSuccess ||
+      ctx->items.ensure(items.size() * sizeof(WorkItem)) != hipSuccess || ctx->queue.ensure(4) != hipSuccess ||
+      ctx->bar.ensure((ls.nbar + 1) * 8) != hipSuccess || ctx->digits.ensure(ls.ndigits * 8) != hipSuccess ||
+      ctx->trace.ensure((ls.ntrace + 1) * 8) != hipSuccess || ctx->out.ensure(sizeof(PairOut)) != hipSuccess)
+    return MGCD_ERR_NOMEM;
+  hipStream_t s = ctx->stream;
+  if (nu) HIP_TRY(hipMemcpyAsync(ctx->limbs.p, u, nu * 4, hipMemcpyHostToDevice, s));
+  if (nv) HIP_TRY(hipMemcpyAsync(ctx->limbs.as<uint32_t>() + nu, v, nv * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(ctx->pairs.p, pd.data(), sizeof(PairDesc), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(ctx->items.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+
+  KernelArgs ka;
+  ka.limbs = ctx->limbs.as<uint32_t>();
+  ka.pairs = ctx->pairs.as<PairDesc>();
+  ka.items = ctx->items.as<WorkItem>();
+  ka.nitems = ls.nitems;
+  ka.L = ctx->L;
+  ka.queue = ctx->queue.as<uint32_t>();
+  ka.mod_u32 = ctx->d_mod_u32.as<uint32_t>();
+  ka.mod_d = ctx->d_mod_d.as<double>();
+  ka.mod_inv = ctx->d_mod_inv.as<double>();
+  ka.bar = ctx->bar.as<unsigned long long>();
+  ka.digits = ctx->digits.as<mgcd_pair_t>();
+  ka.trace = ctx->trace.as<mgcd_pair_t>();
+  ka.out = ctx->out.as<PairOut>();
+  ka.spin_limit = kSpinLimit;
+  rc = launch(ctx, ls, ka);
+  if (rc != MGCD_OK) return rc;
+  PairOut po;
+  HIP_TRY(hipMemcpyAsync(&po, ctx->out.p, sizeof(PairOut), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_kernel_s = ms * 1e-3;
+  if (nt) *nt = po.ntrace;
+  if (nd) *nd = po.ndig;
+  if (trace_cap && po.ntrace)
+    HIP_TRY(hipMemcpy(pivots, ctx->trace.p, std::min<size_t>(po.ntrace, trace_cap) * 8, hipMemcpyDeviceToHost));
+  if (po.status != KST_OK) return kst_to_status(po.status);
+  if (po.ndig > nq) return MGCD_ERR_KERNEL;
+  std::vector<mgcd_pair_t> dig(po.ndig);
+  if (po.ndig) HIP_TRY(hipMemcpy(dig.data(), ctx->digits.p, po.ndig * 8, hipMemcpyDeviceToHost));
+  if (digits && dcap) std::memcpy(digits, dig.data(), std::min<size_t>(dcap, dig.size()) * 8);
+  std::vector<uint32_t> mag;
+  digits_to_limbs(dig.data(), dig.size(), mag);
+  *ng = mag.size();
+  if (mag.size() > cap) return MGCD_ERR_CAPACITY;
+  if (!mag.empty()) std::memcpy(g, mag.data(), mag.size() * 4);
+  return MGCD_OK;
+}
+
+int mgcd_gcd_limbs(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint32_t* g,
+                   size_t cap, size_t* ng) {
+  if (!ctx || (nu && !u) || (nv && !v) || !ng) return MGCD_ERR_ARG;
+  nu = trim(u, nu);
+  nv = trim(v, nv);
+  if (cmp_mag(u, nu, v, nv) < 0) {  // larger operand first (GmpCudaDevice-gcd.cu:61-62)
+    std::swap(u, v);
+    std::swap(nu, nv);
+  }
+  if (nv == 0) {  // gcd(u, 0) = |u|; the reference kernel cannot take v = 0
+    *ng = nu;
+    if (nu > cap) return MGCD_ERR_CAPACITY;
+    if (nu) std::memcpy(g, u, nu * 4);
+    return MGCD_OK;
+  }
+  return mgcd_gcd_limbs_trace(ctx, u, nu, v, nv, g, cap, ng, nullptr, 0, nullptr, nullptr, 0, nullptr);
+}
+
+double mgcd_ctx_last_kernel_seconds(const mgcd_ctx* ctx) { return ctx ? ctx->last_kernel_s : 0.0; }
+
+int mgcd_ctx_synchronize(mgcd_ctx* ctx) {
+  if (!ctx) return MGCD_ERR_ARG;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipDeviceSynchronize());
+  return MGCD_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batches.
+struct mgcd_batch {
+  mgcd_ctx* ctx = nullptr;
+  size_t npairs = 0;
+  std::vector<PairDesc> pd;          // kernel pairs
+  std::vector<int64_t> kidx;         // pair i -> kernel pair index, or -1 (handled on host)
+  std::vector<std::vector<uint32_t>> host_result;  // results of pairs handled on host / converted
+  std::vector<int> status;
+  std::vector<uint32_t> nv_limbs;    // capacity hint per pair
+  std::vector<WorkItem> items;
+  LaunchSet ls;
+  DevBuf limbs, pairs, items_d, queue, bar, digits, trace, out;
+  std::vector<PairOut> h_out;
+  std::vector<mgcd_pair_t> h_digits;
+  double device_s = 0.0, host_s = 0.0;
+  bool ran = false;
+};
+
+extern "C" {
+
+int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, const uint32_t* nu,
+                      const uint64_t* v_off, const uint32_t* nv, size_t npairs, int opts_workgroup, int opts_slots,
+                      mgcd_batch** out) {
+  if (!ctx || !out || (npairs && (!limbs || !u_off || !nu || !v_off || !nv))) return MGCD_ERR_ARG;
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  mgcd_batch* b = new mgcd_batch();
+  b->ctx = ctx;
+  b->npairs = npairs;
+  b->kidx.assign(npairs, -1);
+  b->host_result.resize(npairs);
+  b->status.assign(npairs, MGCD_OK);
+  // Pack the kernel pairs' limbs, larger operand first.
+  std::vector<uint32_t> packed;
+  uint32_t max_nq = 0;
+  for (size_t i = 0; i < npairs; ++i) {
+    const uint32_t* a = limbs + u_off[i];
+    const uint32_t* c = limbs + v_off[i];
+    size_t na = trim(a, nu[i]), nc = trim(c, nv[i]);
+    if (cmp_mag(a, na, c, nc) < 0) {
+      std::swap(a, c);
+      std::swap(na, nc);
+    }
+    if (nc == 0) {  // gcd(u, 0) = |u|
+      b->host_result[i].assign(a, a + na);
+      continue;
+    }
+    const int nqi = mgcd_num_moduli_needed(bit_length(a, na), ctx->L);
+    if (nqi <= 0 || (size_t)nqi > ctx->moduli.size()) {
+      b->status[i] = MGCD_ERR_NOT_ENOUGH_MODULI;
+      continue;
+    }
+    PairDesc p;
+    std::memset(&p, 0, sizeof p);
+    p.u_off = packed.size();
+    packed.insert(packed.end(), a, a + na);
+    p.v_off = packed.size();
+    packed.insert(packed.end(), c, c + nc);
+    p.nu = (uint32_t)na;
+    p.nv = (uint32_t)nc;
+    p.nq = (uint32_t)nqi;
+    max_nq = std::max(max_nq, p.nq);
+    b->kidx[i] = (int64_t)b->pd.size();
+    b->pd.push_back(p);
+  }
+  // Tiling: explicit, or a default that fits the median pair in one workgroup.
+  int T = opts_workgroup, K = opts_slots;
+  if (T <= 0 || K <= 0) {
+    std::vector<uint32_t> nqs;
+    for (auto& p : b->pd) nqs.push_back(p.nq);
+    uint32_t med = 256;
+    if (!nqs.empty()) {
+      std::nth_element(nqs.begin(), nqs.begin() + nqs.size() / 2, nqs.end());
+      med = nqs[nqs.size() / 2];
+    }
+    T = 256;
+    K = 1;
+    while (K < 16 && (uint32_t)(T * K) < med) K *= 2;
+  }
+  const int vi = find_kernel_variant(T, K, ctx->wide);
+  int rc = vi < 0 ? MGCD_ERR_ARG : MGCD_OK;
+  if (rc == MGCD_OK && !b->pd.empty()) rc = plan_items(ctx, vi, b->pd, b->items, b->ls, 0);
+  if (rc == MGCD_OK && !b->pd.empty()) {
+    const size_t np = b->pd.size();
+    if (b->limbs.ensure(packed.size() * 4 + 4) != hipSuccess || b->pairs.ensure(np * sizeof(PairDesc)) != hipSuccess ||
+        b->items_d.ensure(b->items.size() * sizeof(WorkItem)) != hipSuccess || b->queue.ensure(4) != hipSuccess ||
+        b->bar.ensure((b->ls.nbar + 1) * 8) != hipSuccess || b->digits.ensure(b->ls.ndigits * 8) != hipSuccess ||
+        b->trace.ensure(8) != hipSuccess || b->out.ensure(np * sizeof(PairOut)) != hipSuccess) {
+      rc = MGCD_ERR_NOMEM;
+    } else if (hipMemcpy(b->limbs.p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(b->pairs.p, b->pd.data(), np * sizeof(PairDesc), hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(b->items_d.p, b->items.data(), b->items.size() * sizeof(WorkItem), hipMemcpyHostToDevice) !=
+                   hipSuccess) {
+      rc = MGCD_ERR_HIP;
+    }
+  }
+  if (rc != MGCD_OK) {
+    mgcd_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return MGCD_OK;
+}
+
+int mgcd_batch_run(mgcd_batch* b) {
+  if (!b) return MGCD_ERR_ARG;
+  mgcd_ctx* ctx = b->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t np = b->pd.size();
+  b->device_s = 0.0;
+  b->host_s = 0.0;
+  if (np) {
+    KernelArgs ka;
+    ka.limbs = b->limbs.as<uint32_t>();
+    ka.pairs = b->pairs.as<PairDesc>();
+    ka.items = b->items_d.as<WorkItem>();
+    ka.nitems = b->ls.nitems;
+    ka.L = ctx->L;
+    ka.queue = b->queue.as<uint32_t>();
+    ka.mod_u32 = ctx->d_mod_u32.as<uint32_t>();
+    ka.mod_d = ctx->d_mod_d.as<double>();
+    ka.mod_inv = ctx->d_mod_inv.as<double>();
+    ka.bar = b->bar.as<unsigned long long>();
+    ka.digits = b->digits.as<mgcd_pair_t>();
+    ka.trace = b->trace.as<mgcd_pair_t>();
+    ka.out = b->out.as<PairOut>();
+    ka.spin_limit = kSpinLimit;
+    int rc = launch(ctx, b->ls, ka);
+    if (rc != MGCD_OK) return rc;
+    b->h_out.resize(np);
+    b->h_digits.resize(b->ls.ndigits);
+    HIP_TRY(hipMemcpyAsync(b->h_out.data(), b->out.p, np * sizeof(PairOut), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(b->h_digits.data(), b->digits.p, b->ls.ndigits * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) b->device_s = ms * 1e-3;
+    ctx->last_kernel_s = b->device_s;
+    // Digit conversion on host threads.
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<size_t> todo;
+    for (size_t i = 0; i < b->npairs; ++i)
+      if (b->kidx[i] >= 0) todo.push_back(i);
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+      for (;;) {
+        const size_t j = next.fetch_add(1);
+        if (j >= todo.size()) break;
+        const size_t i = todo[j];
+        const PairDesc& p = b->pd[b->kidx[i]];
+        const PairOut& o = b->h_out[b->kidx[i]];
+        if (o.status != KST_OK || o.ndig > p.nq) {
+          b->status[i] = o.status != KST_OK ? kst_to_status(o.status) : MGCD_ERR_KERNEL;
+          b->host_result[i].clear();
+          continue;
+        }
+        b->status[i] = MGCD_OK;
+        digits_to_limbs(b->h_digits.data() + p.dig_off, o.ndig, b->host_result[i]);
+      }
+    };
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (todo.size() < 64) nth = 1;
+    std::vector<std::thread> th;
+    for (unsigned k = 1; k < nth; ++k) th.emplace_back(work);
+    work();
+    for (auto& x : th) x.join();
+    b->host_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  b->ran = true;
+  return MGCD_OK;
+}
+
+int mgcd_batch_status(const mgcd_batch* b, size_t i) {
+  if (!b || i >= b->npairs) return MGCD_ERR_ARG;
+  if (!b->ran && b->kidx[i] >= 0) return MGCD_ERR_KERNEL;
+  return b->status[i];
+}
+
+int mgcd_batch_result(const mgcd_batch* b, size_t i, uint32_t* g, size_t cap, size_t* ng) {
+  if (!b || i >= b->npairs || !ng) return MGCD_ERR_ARG;
+  const int st = mgcd_batch_status(b, i);
+  if (st != MGCD_OK) return st;
+  const auto& r = b->host_result[i];
+  *ng = r.size();
+  if (r.size() > cap) return MGCD_ERR_CAPACITY;
+  if (!r.empty()) std::memcpy(g, r.data(), r.size() * 4);
+  return MGCD_OK;
+}
+
+int mgcd_batch_timing(const mgcd_batch* b, double* device_s, double* host_s) {
+  if (!b) return MGCD_ERR_ARG;
+  if (device_s) *device_s = b->device_s;
+  if (host_s) *host_s = b->host_s;
+  return MGCD_OK;
+}
+
+int mgcd_batch_pair_stats(const mgcd_batch* b, size_t i, uint32_t* nq, uint32_t* iters, uint32_t* digits) {
+  if (!b || i >= b->npairs) return MGCD_ERR_ARG;
+  uint32_t q = 0, r = 0, k = 0;
+  if (b->kidx[i] >= 0) {
+    q = b->pd[b->kidx[i]].nq;
+    if (b->ran) {
+      r = b->h_out[b->kidx[i]].ntrace;
+      k = b->h_out[b->kidx[i]].ndig;
+    }
+  }
+  if (nq) *nq = q;
+  if (iters) *iters = r;
+  if (digits) *digits = k;
+  return MGCD_OK;
+}
+
+int mgcd_batch_tiling(const mgcd_batch* b, int* workgroup, int* slots) {
+  if (!b) return MGCD_ERR_ARG;
+  if (b->ls.variant < 0) {
+    if (workgroup) *workgroup = 0;
+    if (slots) *slots = 0;
+    return MGCD_OK;
+  }
+  const KernelVariant& kv = kernel_variant(b->ls.variant);
+  if (workgroup) *workgroup = kv.T;
+  if (slots) *slots = kv.K;
+  return MGCD_OK;
+}
+
+int mgcd_batch_destroy(mgcd_batch* b) {
+  if (!b) return MGCD_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  for (DevBuf* d : {&b->limbs, &b->pairs, &b->items_d, &b->queue, &b->bar, &b->digits, &b->trace, &b->out})
+    d->release();
+  delete b;
+  return MGCD_OK;
+}
+
+// ---------------------------------------------------------------------------
+int mgcd_certify_quasi_quotient(mgcd_ctx* ctx, int mode, int L, uint64_t* failures, uint64_t* checked) {
+  if (!ctx || !failures || !checked || (mode != 0 && mode != 1) || (mode == 1 && (L < 2 || L > 32)))
+    return MGCD_ERR_ARG;
+  HIP_TRY(hipSetDevice(ctx->device));
+  DevBuf cnt;
+  if (cnt.ensure(16) != hipSuccess) return MGCD_ERR_NOMEM;
+  unsigned long long* d = cnt.as<unsigned long long>();
+  hipStream_t s = ctx->stream;
+  int rc = MGCD_OK;
+  auto run = [&]() -> int {
+    HIP_TRY(launch_fill_u64(d, 2, 0ull, s));
+    const int grid = ctx->num_cus * 16;
+    if (mode == 0) {
+      HIP_TRY(launch_certify_float(d, d + 1, grid, s));
+    } else {
+      for (int b = 0; b < L; ++b) {
+        const uint64_t lo = 1ull << b;           // octave [2^b, 2^(b+1)) of y
+        const uint64_t hi = 1ull << (b + 1);      // b + 1 <= L: all y < 2^L
+        HIP_TRY(launch_certify_endpoints(L, lo, hi, d, d + 1, grid, s));
+      }
+    }
+    unsigned long long h[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *failures = h[0];
+    *checked = h[1];
+    return MGCD_OK;
+  };
+  rc = run();
+  cnt.release();
+  return rc;
+}
+
+}  // extern "C"

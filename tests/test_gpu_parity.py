@@ -1,0 +1,123 @@
+"""GPU parity: the HIP path against the oracle and GMP, through the C ABI.
+
+The trace entry point returns the kernel's pivot sequence (p, b) and its
+signed mixed-radix digits; they must equal the oracle's restatement of the
+reference kernel element by element, and the gcd must equal mpz_gcd.
+"""
+import math
+import random
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_trace(ctx, oracle, mgcd, u, v, L):
+    nq = mgcd.num_moduli_needed(max(u.bit_length(), 2), L)
+    st, g, digits, pivots = ctx.gcd_trace(u, v)
+    ost, od, ot = oracle.gcd_kernel(u, v, ctx.moduli()[:nq], L)
+    status_map = {oracle.OK: mgcd.OK, oracle.REDUX_ERROR: mgcd.ERR_REDUX, oracle.RECOVERY_ERROR: mgcd.ERR_RECOVERY}
+    assert st == status_map[ost], (st, ost)
+    assert pivots == ot, f"pivots differ (first mismatch at {next(i for i, (a, b) in enumerate(zip(pivots, ot)) if a != b) if any(a != b for a, b in zip(pivots, ot)) else min(len(pivots), len(ot))})"
+    if st == mgcd.OK:
+        assert digits == od
+        assert g == oracle.mpz_gcd(u, v) == math.gcd(u, v)
+    return len(pivots), len(digits)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_trace_small_L32(ctx_for, oracle, mgcd, wl, seed):
+    u, v = wl.single_small(seed)
+    R, k = _check_trace(ctx_for(32), oracle, mgcd, u, v, 32)
+    assert R > 100 and k > 10
+
+
+@pytest.mark.parametrize("L", [27, 22])
+def test_trace_small_other_widths(ctx_for, oracle, mgcd, wl, L):
+    u, v = wl.single_small(0)
+    _check_trace(ctx_for(L), oracle, mgcd, u, v, L)
+
+
+@pytest.mark.parametrize("L", [32, 27, 22])
+def test_edge_pairs(ctx_for, oracle, mgcd, wl, L):
+    ctx = ctx_for(L)
+    coprime, equal, vzero = wl.single_small_edges(0)
+    for u, v in (coprime, equal):
+        _check_trace(ctx, oracle, mgcd, u, v, L)
+    # v == 0: the reference kernel runs out of moduli (oracle agrees); the
+    # product path returns |u| like mpz_gcd.
+    u, v = vzero
+    _check_trace(ctx, oracle, mgcd, u, v, L)
+    assert ctx.gcd(u, 0) == u
+    assert ctx.gcd(0, u) == u
+    assert ctx.gcd(0, 0) == 0
+
+
+def test_trace_multi_workgroup(ctx_for, oracle, mgcd):
+    # N_Q = 2304 > 1024: the pair is split over 9 workgroups that exchange
+    # their minima every iteration through the slot barrier.
+    rnd = random.Random(11)
+    g = rnd.getrandbits(3000) | (1 << 2999)
+    u = g * (rnd.getrandbits(17000) | (1 << 16999))
+    v = g * (rnd.getrandbits(17000) | (1 << 16999))
+    if u < v:
+        u, v = v, u
+    _check_trace(ctx_for(32), oracle, mgcd, u, v, 32)
+
+
+@pytest.mark.parametrize("L", [32, 22])
+def test_small_and_ragged_sizes(ctx_for, mgcd, L):
+    ctx = ctx_for(L)
+    rnd = random.Random(5 + L)
+    cases = [(1, 1), (2, 1), (6, 4), (2**32 - 5, 2**31 + 11), (2**64 + 13, 2**63), (12, 18)]
+    for bits in (33, 64, 65, 100, 500, 1000, 1500, 3000, 5000):
+        gb = rnd.randrange(1, bits)
+        g = rnd.getrandbits(gb) | 1
+        cases.append((g * rnd.getrandbits(bits), g * rnd.getrandbits(bits) + g))
+    for u, v in cases:
+        assert ctx.gcd(u, v) == math.gcd(u, v), (u.bit_length(), v.bit_length())
+
+
+@pytest.mark.parametrize("L", [32, 27, 22])
+def test_single_large_1M(ctx_for, oracle, wl, L):
+    u, v = wl.single_large(0)
+    g = ctx_for(L).gcd(u, v)
+    assert g == oracle.mpz_gcd(u, v)
+    assert g.bit_length() >= 100000
+
+
+def test_batch_mixed_small(ctx_for, mgcd, wl):
+    ctx = ctx_for(32)
+    pairs = wl.mixed(n=160, seed=9, lo=64, hi=30000)
+    pairs += [(0, 0), (5, 0), (0, 7), (2**100, 2**100)]
+    b = mgcd.Batch.from_pairs(ctx, pairs)
+    b.run()
+    res = b.results()
+    for i, (u, v) in enumerate(pairs):
+        assert res[i] == math.gcd(u, v), i
+    # the same pairs alone through the one-pair path give the same results
+    for i in (0, 7, 50, 100, 150):
+        assert ctx.gcd(*pairs[i]) == res[i]
+    b.close()
+
+
+def test_batch_alone_vs_mixed(ctx_for, mgcd, wl):
+    ctx = ctx_for(27)
+    mix = wl.mixed(n=64, seed=21, lo=1000, hi=40000)
+    probe = mix[10]
+    alone = mgcd.Batch.from_pairs(ctx, [probe])
+    alone.run()
+    together = mgcd.Batch.from_pairs(ctx, mix)
+    together.run()
+    assert alone.result(0) == together.result(10) == math.gcd(*probe)
+
+
+@pytest.mark.parametrize("T,K", [(256, 1), (256, 4), (1024, 2)])
+def test_batch_tilings_agree(ctx_for, mgcd, wl, T, K):
+    ctx = ctx_for(32)
+    pairs = wl.headline(24, seed=1, nbits=12000, gbits=1200)
+    b = mgcd.Batch.from_pairs(ctx, pairs, workgroup=T, slots=K)
+    assert b.tiling() == (T, K)
+    b.run()
+    for i, (u, v) in enumerate(pairs):
+        assert b.result(i) == math.gcd(u, v)
