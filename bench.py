@@ -113,10 +113,14 @@ def main():
         lat.append(time.perf_counter() - t)
         assert g == results[i], f"pair {i}: one-per-launch result differs from batched"
     lat.sort()
-    p50 = statistics.median(lat)
-    p99 = lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]
-    single_rate = 1.0 / statistics.mean(lat)
-    log(f"one-per-launch: p50 {p50*1e3:.2f} ms p99 {p99*1e3:.2f} ms -> {single_rate:.1f} GCD/s")
+    if lat:
+        p50 = statistics.median(lat)
+        p99 = lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]
+        single_rate = 1.0 / statistics.mean(lat)
+        log(f"one-per-launch: p50 {p50*1e3:.2f} ms p99 {p99*1e3:.2f} ms -> {single_rate:.1f} GCD/s")
+    else:
+        p50 = p99 = float("nan")
+        single_rate = float("nan")
 
     cpu = None
     exact_checked = 0
@@ -164,7 +168,9 @@ def main():
                      "traffic": None, "kernel_ms": round(kern_s * 1e3, 3)},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line))
+    print(json.dumps(line), flush=True)
+    batch.close()
+    ctx.close()
 
 
 if __name__ == "__main__":

@@ -78,6 +78,17 @@ int mgcd_ctx_L(const mgcd_ctx* ctx);
 int mgcd_ctx_num_moduli(const mgcd_ctx* ctx);           /* table size */
 int mgcd_ctx_max_grid(const mgcd_ctx* ctx);             /* getMaxGridSize (GmpCuda.h:118) */
 int mgcd_ctx_moduli(const mgcd_ctx* ctx, uint32_t* out, size_t cap); /* copy of the table */
+/* Context options (mgcd_ctx_set_option). */
+enum {
+  /* N_Q for every following pair, 0 (default) for the reference's sizing. */
+  MGCD_OPT_FORCE_MODULI = 1,
+  /* 1 (default): where the reference's run stops with "Ran out of moduli"
+   * (reduction or recovery), mgcd_gcd_limbs and batches retry the pair with
+   * a quarter more moduli, up to the whole table; 0: report the error.
+   * mgcd_gcd_limbs_trace never retries. */
+  MGCD_OPT_RETRY = 2
+};
+int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value);
 /* Blocks until all work queued on the context's device has finished. */
 int mgcd_ctx_synchronize(mgcd_ctx* ctx);
 /* Seconds the last gcd kernel launch of this context took (HIP events). */

@@ -21,9 +21,13 @@
 //
 // The bias makes q~ never exceed floor(x/y) (the combined relative error of
 // the conversions, v_rcp_f32 and two roundings is below 2^-21), so every
-// remainder x - q~ y is non-negative; it is computed exactly with an FMA
-// (the exact integer correction).  When q~ falls short the remainder lands
-// in [y, 2y) and the next step has quotient 0, which only swaps the pair.
+// remainder x - q~ y is non-negative; it is computed exactly with an FMA,
+// and one exact integer correction (r >= y: r -= y) makes the quotient
+// exact whenever q~ >= floor(x/y) - 1.  The correction is what guarantees
+// progress: without it, x and y that round to the same float give q~ = 0
+// and the pair would only swap, forever.  When floor(x/y) >= 2^19 the
+// estimate may fall short by more; the remainder then stays >= y, the next
+// step has quotient 0 (a swap) and division resumes, still exact.
 // The "never exceeds" postcondition, and "short by at most one when
 // floor(x/y) < 2^19", are certified exhaustively on the device by
 // mgcd_certify_quasi_quotient (float domain x < 2^23, y < 2^22 point by
@@ -79,8 +83,12 @@ struct Arith32 {
     float f0 = (float)M.m, f1 = (float)y;
     while (r1 > 1.0) {
       const double q = (double)quasi_quo_f(f0, f1);
-      const double r2 = __builtin_fma(-q, r1, r0);
-      const double s2 = __builtin_fma(-q, s1, s0);
+      double r2 = __builtin_fma(-q, r1, r0);
+      double s2 = __builtin_fma(-q, s1, s0);
+      if (r2 >= r1) {  // exact integer correction: q~ = floor(r0/r1) - 1
+        r2 -= r1;
+        s2 -= s1;
+      }
       r0 = r1; f0 = f1; s0 = s1;
       r1 = r2; s1 = s2; f1 = (float)r2;
     }
@@ -102,8 +110,12 @@ struct Arith22 {
     float r0 = m, r1 = (float)y, s0 = 0.0f, s1 = 1.0f;
     while (r1 > 1.0f) {
       const float q = quasi_quo_f(r0, r1);
-      const float r2 = __builtin_fmaf(-q, r1, r0);
-      const float s2 = __builtin_fmaf(-q, s1, s0);
+      float r2 = __builtin_fmaf(-q, r1, r0);
+      float s2 = __builtin_fmaf(-q, s1, s0);
+      if (r2 >= r1) {  // exact integer correction
+        r2 -= r1;
+        s2 -= s1;
+      }
       r0 = r1; s0 = s1;
       r1 = r2; s1 = s2;
     }

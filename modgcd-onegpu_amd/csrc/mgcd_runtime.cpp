@@ -179,6 +179,8 @@ struct mgcd_ctx {
   // pooled buffers of the single-pair path
   DevBuf limbs, pairs, items, queue, bar, digits, trace, out;
   double last_kernel_s = 0.0;
+  uint32_t force_nq = 0;  // MGCD_OPT_FORCE_MODULI
+  bool retry = true;      // MGCD_OPT_RETRY
 };
 
 // A prepared launch over device-resident pairs.
@@ -403,19 +405,40 @@ int mgcd_ctx_moduli(const mgcd_ctx* ctx, uint32_t* out, size_t cap) {
 
 const char* mgcd_last_error_detail(void) { return g_last_error.c_str(); }
 
-int mgcd_gcd_limbs_trace(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint32_t* g,
-                         size_t cap, size_t* ng, mgcd_pair_t* digits, size_t dcap, size_t* nd,
-                         mgcd_pair_t* pivots, size_t tcap, size_t* nt) {
-  if (!ctx || (nu && !u) || (nv && !v) || !ng) return MGCD_ERR_ARG;
-  HIP_TRY(hipSetDevice(ctx->device));
-  nu = trim(u, nu);
-  nv = trim(v, nv);
+int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value) {
+  if (!ctx) return MGCD_ERR_ARG;
+  switch (option) {
+    case MGCD_OPT_FORCE_MODULI:
+      if (value < 0 || (uint64_t)value > ctx->moduli.size()) return MGCD_ERR_ARG;
+      ctx->force_nq = (uint32_t)value;
+      return MGCD_OK;
+    case MGCD_OPT_RETRY:
+      ctx->retry = value != 0;
+      return MGCD_OK;
+    default:
+      return MGCD_ERR_ARG;
+  }
+}
+
+}  // extern "C"
+
+namespace {
+
+// N_Q for an operand of `bits` bits: the reference's sizing, or the forced
+// count.  A 1-bit operand is sized as a 2-bit one: the reference's formula
+// divides by ln(1) = 0 there (GmpCudaDevice-gcd.cu:54).
+int moduli_for(const mgcd_ctx* ctx, uint64_t bits) {
+  if (ctx->force_nq) return (int)ctx->force_nq;
+  return mgcd_num_moduli_needed(std::max<uint64_t>(bits, 2), ctx->L);
+}
+
+// One pair, u taken as given, with N_Q = nq moduli.
+int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, int nqi, uint32_t* g,
+               size_t cap, size_t* ng, mgcd_pair_t* digits, size_t dcap, size_t* nd, mgcd_pair_t* pivots,
+               size_t tcap, size_t* nt) {
   if (nd) *nd = 0;
   if (nt) *nt = 0;
   *ng = 0;
-  const uint64_t bits = bit_length(u, nu);
-  if (bits == 0) return MGCD_ERR_ARG;  // the reference's sizing divides by ln(1)
-  const int nqi = mgcd_num_moduli_needed(bits, ctx->L);
   if (nqi <= 0 || (size_t)nqi > ctx->moduli.size()) return MGCD_ERR_NOT_ENOUGH_MODULI;
   const uint32_t nq = (uint32_t)nqi;
   const int vi = choose_single_variant(ctx, nq);
@@ -484,9 +507,26 @@ int mgcd_gcd_limbs_trace(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint
   return MGCD_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int mgcd_gcd_limbs_trace(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint32_t* g,
+                         size_t cap, size_t* ng, mgcd_pair_t* digits, size_t dcap, size_t* nd,
+                         mgcd_pair_t* pivots, size_t tcap, size_t* nt) {
+  if (!ctx || (nu && !u) || (nv && !v) || !ng) return MGCD_ERR_ARG;
+  HIP_TRY(hipSetDevice(ctx->device));
+  nu = trim(u, nu);
+  nv = trim(v, nv);
+  const uint64_t bits = bit_length(u, nu);
+  if (bits == 0) return MGCD_ERR_ARG;
+  return gcd_single(ctx, u, nu, v, nv, moduli_for(ctx, bits), g, cap, ng, digits, dcap, nd, pivots, tcap, nt);
+}
+
 int mgcd_gcd_limbs(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint32_t* g,
                    size_t cap, size_t* ng) {
   if (!ctx || (nu && !u) || (nv && !v) || !ng) return MGCD_ERR_ARG;
+  HIP_TRY(hipSetDevice(ctx->device));
   nu = trim(u, nu);
   nv = trim(v, nv);
   if (cmp_mag(u, nu, v, nv) < 0) {  // larger operand first (GmpCudaDevice-gcd.cu:61-62)
@@ -499,7 +539,15 @@ int mgcd_gcd_limbs(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* 
     if (nu) std::memcpy(g, u, nu * 4);
     return MGCD_OK;
   }
-  return mgcd_gcd_limbs_trace(ctx, u, nu, v, nv, g, cap, ng, nullptr, 0, nullptr, nullptr, 0, nullptr);
+  // The reference's attempt first; where it would run out of moduli, retry
+  // with a quarter more (up to the whole table) when retries are enabled.
+  int nq = moduli_for(ctx, bit_length(u, nu));
+  for (;;) {
+    const int rc = gcd_single(ctx, u, nu, v, nv, nq, g, cap, ng, nullptr, 0, nullptr, nullptr, 0, nullptr);
+    const int table = (int)ctx->moduli.size();
+    if ((rc != MGCD_ERR_REDUX && rc != MGCD_ERR_RECOVERY) || !ctx->retry || ctx->force_nq || nq >= table) return rc;
+    nq = std::min(table, ((nq + nq / 4 + 255) / 256) * 256);
+  }
 }
 
 double mgcd_ctx_last_kernel_seconds(const mgcd_ctx* ctx) { return ctx ? ctx->last_kernel_s : 0.0; }
@@ -530,6 +578,7 @@ struct mgcd_batch {
   std::vector<mgcd_pair_t> h_digits;
   double device_s = 0.0, host_s = 0.0;
   bool ran = false;
+  std::vector<uint32_t> packed;      // host copy of the kernel pairs' limbs (for retries)
 };
 
 extern "C" {
@@ -547,7 +596,7 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
   b->host_result.resize(npairs);
   b->status.assign(npairs, MGCD_OK);
   // Pack the kernel pairs' limbs, larger operand first.
-  std::vector<uint32_t> packed;
+  std::vector<uint32_t>& packed = b->packed;
   uint32_t max_nq = 0;
   for (size_t i = 0; i < npairs; ++i) {
     const uint32_t* a = limbs + u_off[i];
@@ -561,7 +610,7 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
       b->host_result[i].assign(a, a + na);
       continue;
     }
-    const int nqi = mgcd_num_moduli_needed(bit_length(a, na), ctx->L);
+    const int nqi = mgcd_num_moduli_needed(std::max<uint64_t>(bit_length(a, na), 2), ctx->L);
     if (nqi <= 0 || (size_t)nqi > ctx->moduli.size()) {
       b->status[i] = MGCD_ERR_NOT_ENOUGH_MODULI;
       continue;
@@ -680,6 +729,27 @@ int mgcd_batch_run(mgcd_batch* b) {
     work();
     for (auto& x : th) x.join();
     b->host_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // Pairs the reference's sizing left short of moduli: retry one by one
+    // with a quarter more moduli each time (MGCD_OPT_RETRY).
+    if (ctx->retry && !ctx->force_nq) {
+      for (size_t i : todo) {
+        if (b->status[i] != MGCD_ERR_REDUX && b->status[i] != MGCD_ERR_RECOVERY) continue;
+        const PairDesc& p = b->pd[b->kidx[i]];
+        const uint32_t* a = b->packed.data() + p.u_off;
+        const uint32_t* c = b->packed.data() + p.v_off;
+        const int table = (int)ctx->moduli.size();
+        int nq = (int)p.nq, rc = b->status[i];
+        std::vector<uint32_t> gl(p.nv + 1);
+        size_t ng = 0;
+        while ((rc == MGCD_ERR_REDUX || rc == MGCD_ERR_RECOVERY) && nq < table) {
+          nq = std::min(table, ((nq + nq / 4 + 255) / 256) * 256);
+          rc = gcd_single(ctx, a, p.nu, c, p.nv, nq, gl.data(), gl.size(), &ng, nullptr, 0, nullptr, nullptr, 0,
+                          nullptr);
+        }
+        b->status[i] = rc;
+        b->host_result[i].assign(gl.begin(), gl.begin() + (rc == MGCD_OK ? ng : 0));
+      }
+    }
   }
   b->ran = true;
   return MGCD_OK;

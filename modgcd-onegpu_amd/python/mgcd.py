@@ -36,8 +36,11 @@ EXPORTS = [
     "mgcd_ctx_moduli", "mgcd_gcd_limbs", "mgcd_gcd_limbs_trace", "mgcd_batch_create", "mgcd_batch_run",
     "mgcd_batch_status", "mgcd_batch_result", "mgcd_batch_timing", "mgcd_batch_tiling", "mgcd_batch_destroy",
     "mgcd_certify_quasi_quotient", "mgcd_last_error_detail", "mgcd_ctx_last_kernel_seconds",
-    "mgcd_ctx_synchronize", "mgcd_batch_pair_stats",
+    "mgcd_ctx_synchronize", "mgcd_batch_pair_stats", "mgcd_ctx_set_option",
 ]
+
+OPT_FORCE_MODULI = 1
+OPT_RETRY = 2
 
 
 class Pair(ctypes.Structure):
@@ -94,6 +97,7 @@ def lib():
         L.mgcd_ctx_last_kernel_seconds.argtypes = [ctypes.c_void_p]
         L.mgcd_ctx_last_kernel_seconds.restype = ctypes.c_double
         L.mgcd_ctx_synchronize.argtypes = [ctypes.c_void_p]
+        L.mgcd_ctx_set_option.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]
         L.mgcd_batch_pair_stats.argtypes = [ctypes.c_void_p, ctypes.c_size_t, P32, P32, P32]
         _lib = L
     return _lib
@@ -199,6 +203,9 @@ class Context:
         d = [(digits[i].modulus, digits[i].value) for i in range(min(nd.value, dcap))]
         t = [(pivots[i].modulus, pivots[i].value) for i in range(min(nt.value, tcap))]
         return st, (limbs_to_int(g[:ng.value]) if st == OK else None), d, t
+
+    def set_option(self, option, value):
+        check(lib().mgcd_ctx_set_option(self.h, option, value), "mgcd_ctx_set_option")
 
     def synchronize(self):
         check(lib().mgcd_ctx_synchronize(self.h), "mgcd_ctx_synchronize")

@@ -86,6 +86,33 @@ def test_single_large_1M(ctx_for, oracle, wl, L):
     assert g.bit_length() >= 100000
 
 
+def test_single_large_1M_L22_reference_sizing(ctx_for, mgcd, wl):
+    # With the reference's N_Q (92,160) the 22-bit run stops in the reduction
+    # loop: at L = 22 each iteration shrinks the pair by about L/2 bits and
+    # the size bookkeeping (getGcdKernel.cu:390-399) runs out after ~80.7k
+    # iterations.  The trace path keeps that behaviour; the product path
+    # above retries with more moduli.
+    u, v = wl.single_large(0)
+    st, g, d, p = ctx_for(22).gcd_trace(u, v)
+    assert st == mgcd.ERR_REDUX and len(p) > 80000
+
+
+def test_forced_moduli_failure_parity(ctx_for, oracle, mgcd, wl):
+    ctx = ctx_for(32)
+    u, v = wl.single_small(3)
+    ctx.set_option(mgcd.OPT_FORCE_MODULI, 256)
+    try:
+        st, g, d, p = ctx.gcd_trace(u, v)
+        ost, od, ot = oracle.gcd_kernel(u, v, ctx.moduli()[:256], 32)
+        assert ost == oracle.REDUX_ERROR and st == mgcd.ERR_REDUX
+        assert p == ot
+        with pytest.raises(mgcd.MgcdError):  # a forced N_Q is never retried
+            ctx.gcd(u, v)
+    finally:
+        ctx.set_option(mgcd.OPT_FORCE_MODULI, 0)
+    assert ctx.gcd(u, v) == math.gcd(u, v)
+
+
 def test_batch_mixed_small(ctx_for, mgcd, wl):
     ctx = ctx_for(32)
     pairs = wl.mixed(n=160, seed=9, lo=64, hi=30000)
