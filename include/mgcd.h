@@ -86,7 +86,11 @@ enum {
    * (reduction or recovery), mgcd_gcd_limbs and batches retry the pair with
    * a quarter more moduli, up to the whole table; 0: report the error.
    * mgcd_gcd_limbs_trace never retries. */
-  MGCD_OPT_RETRY = 2
+  MGCD_OPT_RETRY = 2,
+  /* Tiling of the one-pair path: threads per workgroup and moduli slots per
+   * thread (both 0, the default: chosen from N_Q). */
+  MGCD_OPT_SINGLE_WORKGROUP = 3,
+  MGCD_OPT_SINGLE_SLOTS = 4
 };
 int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value);
 /* Blocks until all work queued on the context's device has finished. */
@@ -116,10 +120,12 @@ int mgcd_gcd_limbs_trace(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint
  * packed in `limbs`; pair i is (limbs + u_off[i], nu[i]) and
  * (limbs + v_off[i], nv[i]).  The inputs are copied to device memory once,
  * at creation; mgcd_batch_run computes every gcd from the resident copy.
- * opts_workgroup / opts_slots: 0 lets the autotuner choose the tiling. */
+ * Tiling: threads per workgroup, moduli slots per thread, pairs per group
+ * (pairs with equal N_Q that share one modular inverse per modulus); all
+ * three 0 lets the library choose. */
 int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, const uint32_t* nu,
                       const uint64_t* v_off, const uint32_t* nv, size_t npairs,
-                      int opts_workgroup, int opts_slots, mgcd_batch** out);
+                      int opts_workgroup, int opts_slots, int opts_group, mgcd_batch** out);
 int mgcd_batch_run(mgcd_batch* b);
 int mgcd_batch_status(const mgcd_batch* b, size_t i);
 int mgcd_batch_result(const mgcd_batch* b, size_t i, uint32_t* g, size_t cap, size_t* ng);
@@ -129,8 +135,9 @@ int mgcd_batch_timing(const mgcd_batch* b, double* device_s, double* host_s);
 /* Work done for pair i in the last run: N_Q, elimination iterations R and
  * recovery digits k (0 for pairs answered on the host). */
 int mgcd_batch_pair_stats(const mgcd_batch* b, size_t i, uint32_t* nq, uint32_t* iters, uint32_t* digits);
-/* The tiling chosen: threads per workgroup, moduli slots per thread. */
-int mgcd_batch_tiling(const mgcd_batch* b, int* workgroup, int* slots);
+/* The tiling chosen: threads per workgroup, moduli slots per thread, pairs
+ * per group. */
+int mgcd_batch_tiling(const mgcd_batch* b, int* workgroup, int* slots, int* group);
 int mgcd_batch_destroy(mgcd_batch* b);
 
 /* Exhaustive certification of the FP32-reciprocal quasi-quotient on this

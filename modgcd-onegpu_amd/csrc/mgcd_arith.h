@@ -76,23 +76,27 @@ struct Arith32 {
     const double q = __builtin_rint(h * M.minv);
     return __builtin_fma(-q, M.m, h) + l;
   }
+  // x <- x mod y (exactly, see quasi_quo_f), sx <- sx - q sy, fx <- fl32(x).
+  static __device__ __forceinline__ void eea_step(double& x, float& fx, double& sx, double y, float fy, double sy) {
+    double q = (double)quasi_quo_f(fx, fy);
+    const double r = __builtin_fma(-q, y, x);
+    q += (r >= y) ? 1.0 : 0.0;  // exact integer correction: q~ was floor(x/y) - 1
+    x = __builtin_fma(-q, y, x);
+    sx = __builtin_fma(-q, sy, sx);
+    fx = (float)x;
+  }
   // y in [1, m) canonical; returns y^-1 mod m in [0, m).
-  // Invariant r_i == s_i * y (mod m); stops at r1 <= 1.
+  // Invariant r_i == s_i * y (mod m); unrolled by two so the pair swaps
+  // roles instead of moving registers; stops when a remainder is <= 1.
   static __device__ double modinv(double y, ModD M) {
-    double r0 = M.m, r1 = y, s0 = 0.0, s1 = 1.0;
+    double r0 = M.m, r1 = y, s0 = 0.0, s1 = 1.0, s;
     float f0 = (float)M.m, f1 = (float)y;
-    while (r1 > 1.0) {
-      const double q = (double)quasi_quo_f(f0, f1);
-      double r2 = __builtin_fma(-q, r1, r0);
-      double s2 = __builtin_fma(-q, s1, s0);
-      if (r2 >= r1) {  // exact integer correction: q~ = floor(r0/r1) - 1
-        r2 -= r1;
-        s2 -= s1;
-      }
-      r0 = r1; f0 = f1; s0 = s1;
-      r1 = r2; s1 = s2; f1 = (float)r2;
+    for (;;) {
+      eea_step(r0, f0, s0, r1, f1, s1);
+      if (!(r0 > 1.0)) { s = (r0 == 1.0) ? s0 : s1; break; }
+      eea_step(r1, f1, s1, r0, f0, s0);
+      if (!(r1 > 1.0)) { s = (r1 == 1.0) ? s1 : s0; break; }
     }
-    const double s = (r1 == 1.0) ? s1 : s0;
     return s < 0.0 ? s + M.m : s;
   }
 };
@@ -105,21 +109,22 @@ struct Arith22 {
     const double q = __builtin_rint(h * M.minv);
     return __builtin_fma(-q, M.m, h);
   }
+  static __device__ __forceinline__ void eea_step(float& x, float& sx, float y, float sy) {
+    float q = quasi_quo_f(x, y);
+    const float r = __builtin_fmaf(-q, y, x);
+    q += (r >= y) ? 1.0f : 0.0f;  // exact integer correction (q < 2^22: the sum is exact)
+    x = __builtin_fmaf(-q, y, x);
+    sx = __builtin_fmaf(-q, sy, sx);
+  }
   static __device__ double modinv(double y, ModD M) {
     const float m = (float)M.m;
-    float r0 = m, r1 = (float)y, s0 = 0.0f, s1 = 1.0f;
-    while (r1 > 1.0f) {
-      const float q = quasi_quo_f(r0, r1);
-      float r2 = __builtin_fmaf(-q, r1, r0);
-      float s2 = __builtin_fmaf(-q, s1, s0);
-      if (r2 >= r1) {  // exact integer correction
-        r2 -= r1;
-        s2 -= s1;
-      }
-      r0 = r1; s0 = s1;
-      r1 = r2; s1 = s2;
+    float r0 = m, r1 = (float)y, s0 = 0.0f, s1 = 1.0f, s;
+    for (;;) {
+      eea_step(r0, s0, r1, s1);
+      if (!(r0 > 1.0f)) { s = (r0 == 1.0f) ? s0 : s1; break; }
+      eea_step(r1, s1, r0, s0);
+      if (!(r1 > 1.0f)) { s = (r1 == 1.0f) ? s1 : s0; break; }
     }
-    const float s = (r1 == 1.0f) ? s1 : s0;
     return (double)(s < 0.0f ? s + m : s);
   }
 };

@@ -15,17 +15,28 @@ struct PairDesc {
   uint64_t v_off;      // limb offset of v
   uint32_t nu, nv;     // limb counts (no leading zero limbs)
   uint32_t nq;         // N_Q: moduli the pair starts with
-  uint32_t G;          // workgroups the pair is split over
+  uint32_t pad0;
   uint64_t dig_off;    // first digit slot in the digit buffer
-  uint64_t bar_off;    // first barrier slot (2*G uint64 slots), if G > 1
   uint64_t trace_off;  // first pivot-trace slot
   uint32_t trace_cap;  // 0: no trace
   uint32_t pad;
 };
 
-// One unit of the persistent work queue: part `part` of pair `pair`.
+// Pairs that share N_Q and are processed together by the same workgroups:
+// pairs [first, first + count) of the descriptor array, count <= B of the
+// kernel variant.  Every lane then holds the same modulus for all of them,
+// which lets it invert all their values with one modular inverse.
+struct GroupDesc {
+  uint32_t first;
+  uint32_t count;
+  uint32_t nq;         // N_Q of every pair of the group
+  uint32_t G;          // workgroups the group is split over
+  uint64_t bar_off;    // first barrier slot (2*G*B uint64 slots), if G > 1
+};
+
+// One unit of the persistent work queue: part `part` of group `group`.
 struct WorkItem {
-  uint32_t pair;
+  uint32_t group;
   uint32_t part;
 };
 
@@ -42,6 +53,7 @@ enum : int32_t { KST_OK = 0, KST_REDUX = 1, KST_RECOVERY = 2, KST_TIMEOUT = 3, K
 struct KernelArgs {
   const uint32_t* limbs;
   const PairDesc* pairs;
+  const GroupDesc* groups;
   const WorkItem* items;
   uint32_t nitems;
   int L;
@@ -56,17 +68,18 @@ struct KernelArgs {
   uint32_t spin_limit;         // polls before a barrier wait gives up
 };
 
-// Kernel variants: threads per workgroup T and moduli slots per thread K,
-// for the two arithmetic classes (L <= 22: FP32 inverse; L <= 32).
+// Kernel variants: threads per workgroup T, moduli slots per thread K,
+// pairs per group B, for the two arithmetic classes (L <= 22: FP32
+// inverse; L <= 32).
 struct KernelVariant {
-  int T, K, wide;  // wide = 1 for the 32-bit class
-  const void* fn;  // __global__ function pointer for hipLaunchCooperativeKernel
+  int T, K, B, wide;  // wide = 1 for the 32-bit class
+  const void* fn;     // __global__ function pointer for hipLaunchCooperativeKernel
 };
 
 int kernel_variant_count();
 const KernelVariant& kernel_variant(int i);
-// Finds the variant with exactly (T, K, wide), or -1.
-int find_kernel_variant(int T, int K, int wide);
+// Finds the variant with exactly (T, K, B, wide), or -1.
+int find_kernel_variant(int T, int K, int B, int wide);
 
 // Small helper kernels.
 hipError_t launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s);

@@ -66,51 +66,93 @@ __device__ __forceinline__ int ffs_abs(int32_t x) {
 
 __device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return b < a ? b : a; }
 
+// A value known to be the same in every lane, moved to scalar registers so
+// that everything derived from it (pivot index, modulus, bookkeeping) stays
+// scalar too.
+__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Wave-wide minimum, result uniform.  Within each row of 16 lanes DPP
+// quad_perm [1,0,3,2], [2,3,0,1] and row_ror 4, 8 leave the row minimum in
+// every lane; the four rows are then combined through readlane in SGPRs.
+// Must be called with all 64 lanes active.
+__device__ __forceinline__ uint32_t dpp_min_u32(uint32_t x) {
+  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0xB1, 0xF, 0xF, false));
+  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x4E, 0xF, 0xF, false));
+  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x124, 0xF, 0xF, false));
+  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x128, 0xF, 0xF, false));
+  const uint32_t r0 = __builtin_amdgcn_readlane(x, 0), r1 = __builtin_amdgcn_readlane(x, 16);
+  const uint32_t r2 = __builtin_amdgcn_readlane(x, 32), r3 = __builtin_amdgcn_readlane(x, 48);
+  return min(min(r0, r1), min(r2, r3));
+}
+
 __device__ __forceinline__ uint64_t wave_min(uint64_t x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x = min64(x, __shfl_xor(x, off, 64));
-  return x;
+  const uint32_t hi = (uint32_t)(x >> 32);
+  const uint32_t mh = dpp_min_u32(hi);
+  const uint32_t ml = dpp_min_u32(hi == mh ? (uint32_t)x : 0xFFFFFFFFu);
+  return ((uint64_t)mh << 32) | ml;
 }
 
 struct Ctx {
-  uint64_t* red;             // LDS [2][NW]
-  uint64_t* bc;              // LDS [2]
+  uint64_t* red;             // LDS [2][B][NW]
+  uint64_t* bc;              // LDS [2][B]
+  int nb;                    // B: per-parity stride of red (B*NW) and bc (B)
   int par;                   // LDS double-buffer parity
-  uint32_t G, g;             // parts of the pair, this part
+  uint32_t G, g;             // parts of the group, this part
   uint32_t rnd;              // cross-workgroup round counter
-  unsigned long long* bar;   // this pair's 2*G slots
+  unsigned long long* bar;   // this group's 2*G*B slots
   uint32_t spin_limit;
 };
 
-// Minimum of x over the workgroup; every thread gets it.  One barrier; the
-// LDS buffer alternates so the next call cannot overwrite values still read.
-template <int T>
-__device__ __forceinline__ uint64_t wg_min(uint64_t x, Ctx& C) {
+// Minima of x[0..N) over the workgroup; every thread gets them.  One
+// barrier; the LDS buffer alternates so the next call cannot overwrite
+// values still being read.
+template <int T, int N>
+__device__ __forceinline__ void wg_min(uint64_t (&x)[N], Ctx& C) {
   constexpr int NW = T / 64;
-  x = wave_min(x);
-  uint64_t* red = C.red + C.par * NW;
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
-  __syncthreads();
-  uint64_t r = red[0];
+  uint64_t* red = C.red + C.par * C.nb * NW;  // fixed stride: calls with other N never overlap
 #pragma unroll
-  for (int w = 1; w < NW; ++w) r = min64(r, red[w]);
+  for (int b = 0; b < N; ++b) {
+    const uint64_t w = wave_min(x[b]);
+    if ((threadIdx.x & 63) == 0) red[b * NW + (threadIdx.x >> 6)] = w;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < N; ++b) {
+    uint64_t r = red[b * NW];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) r = min64(r, red[b * NW + w]);
+    x[b] = uniform64(r);
+  }
   C.par ^= 1;
-  return r;
 }
 
-// Minimum over the G workgroups of the pair (x: this workgroup's minimum,
-// the same in every thread).  Returns NONE on timeout.
-__device__ uint64_t grid_min(uint64_t x, Ctx& C) {
+// Minima of x[0..N) over the G workgroups of the group (x: this workgroup's
+// minima, uniform).  Slot layout of a row: [G][N].  Sets every x[b] to NONE
+// on timeout.
+template <int N>
+__device__ void grid_min(uint64_t (&x)[N], Ctx& C) {
   C.rnd += 1;
   const uint64_t tag = (uint64_t)((C.rnd - 1) % 4095u + 1u);
-  unsigned long long* row = C.bar + (size_t)(C.rnd & 1) * C.G;
-  if (threadIdx.x == 0)
-    __hip_atomic_store(row + C.g, (unsigned long long)((tag << 52) | (x & PAYLOAD)), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long* row = C.bar + (size_t)(C.rnd & 1) * C.G * N;
+  if (threadIdx.x < N) {
+    uint64_t v = x[0];
+#pragma unroll
+    for (int b = 1; b < N; ++b)
+      if ((int)threadIdx.x == b) v = x[b];
+    __hip_atomic_store(row + (size_t)C.g * N + threadIdx.x, (unsigned long long)((tag << 52) | (v & PAYLOAD)),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   if (threadIdx.x < 64) {
-    uint64_t m = NONE;
+    uint64_t m[N];
+#pragma unroll
+    for (int b = 0; b < N; ++b) m[b] = NONE;
     bool late = false;
-    for (uint32_t j = threadIdx.x; j < C.G; j += 64) {
+    const uint32_t nslots = C.G * N;
+    for (uint32_t j = threadIdx.x; j < nslots; j += 64) {
       uint64_t v;
       uint32_t spins = 0;
       for (;;) {
@@ -119,16 +161,22 @@ __device__ uint64_t grid_min(uint64_t x, Ctx& C) {
         if (++spins > C.spin_limit) { late = true; break; }
         __builtin_amdgcn_s_sleep(1);
       }
-      m = min64(m, v & PAYLOAD);
+      const uint32_t b = j % N;
+#pragma unroll
+      for (int q = 0; q < N; ++q)
+        if ((uint32_t)q == b) m[q] = min64(m[q], v & PAYLOAD);
     }
-    m = wave_min(m);
     const bool any_late = __ballot(late) != 0;
-    if (threadIdx.x == 0) C.bc[C.par] = any_late ? NONE : m;
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+      const uint64_t r = wave_min(m[b]);
+      if (threadIdx.x == 0) C.bc[C.par * C.nb + b] = any_late ? NONE : r;
+    }
   }
   __syncthreads();
-  const uint64_t r = C.bc[C.par];
+#pragma unroll
+  for (int b = 0; b < N; ++b) x[b] = uniform64(C.bc[C.par * C.nb + b]);
   C.par ^= 1;
-  return r;
 }
 
 // (a) Residues of x (n limbs) for every slot row in use.
@@ -152,21 +200,48 @@ __device__ void residues(const uint32_t* __restrict__ x, uint32_t n, double (&r)
   }
 }
 
-template <int T, int K, class A>
-__device__ void process_item(const KernelArgs& args, const PairDesc& P, uint32_t pair, double* tile, Ctx& C,
-                             uint64_t* sel_lds) {
+// y_b^-1 for y[0..B) (each canonical or signed reduced, nonzero mod m) with
+// one modular inverse: prefix products p_b = y_0 ... y_b, inv = p_{B-1}^-1,
+// then y_b^-1 = inv * p_{b-1} and inv <- inv * y_b going down (Montgomery's
+// batch-inversion trick).  Results signed reduced.
+template <int B, class A>
+__device__ __forceinline__ void batch_inverse(const double (&y)[B], double (&inv)[B], ModD M) {
+  if (B == 1) {
+    inv[0] = A::modinv(canon(y[0], M), M);
+    return;
+  }
+  double p[B];
+  p[0] = y[0];
+#pragma unroll
+  for (int b = 1; b < B; ++b) p[b] = A::mulmod(p[b - 1], y[b], M);
+  double r = A::modinv(canon(p[B - 1], M), M);
+#pragma unroll
+  for (int b = B - 1; b >= 1; --b) {
+    inv[b] = A::mulmod(r, p[b - 1], M);
+    r = A::mulmod(r, y[b], M);
+  }
+  inv[0] = r;
+}
+
+// Everything one workgroup does for part C.g of a group of up to B pairs.
+// Pairs of a group run the elimination in lockstep (a pair that has left
+// the loop only skips its arithmetic); recovery then runs pair by pair.
+template <int T, int K, int B, class A>
+__device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, double* tile, Ctx& C,
+                              uint64_t* sel_lds) {
   const uint32_t t = threadIdx.x;
   const uint32_t g = C.g;
+  const uint32_t nq = Gd.nq;
   // Part g owns moduli [g*nq/G, (g+1)*nq/G): sizes differ by at most one,
   // none is empty (G <= nq), none exceeds T*K (checked by the host).
-  const uint32_t base = (uint32_t)(((uint64_t)g * P.nq) / C.G);
-  const uint32_t cnt = (uint32_t)(((uint64_t)(g + 1) * P.nq) / C.G) - base;
+  const uint32_t base = (uint32_t)(((uint64_t)g * nq) / C.G);
+  const uint32_t cnt = (uint32_t)(((uint64_t)(g + 1) * nq) / C.G) - base;
   const int rows = (int)((cnt + T - 1) / T);
   const bool leader = (g == 0) && (t == 0);
   const int L = args.L;
+  const uint32_t npairs = Gd.count;
 
   ModD M[K];
-  double a[K], c[K], s[K];
   uint32_t valid = 0;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -179,122 +254,203 @@ __device__ void process_item(const KernelArgs& args, const PairDesc& P, uint32_t
       M[k].m = 3.0;  // unused lane of a partial row: any modulus keeps it harmless
       M[k].minv = 1.0 / 3.0;
     }
-    s[k] = 1.0;
   }
-  uint32_t active = valid;
 
-  residues<T, K>(args.limbs + P.u_off, P.nu, a, M, rows, tile);
-  residues<T, K>(args.limbs + P.v_off, P.nv, c, M, rows, tile);
-
-  int32_t status = KST_OK;
-  uint32_t ntrace = 0, ndig = 0;
-  int32_t remaining = (int32_t)P.nq;             // totalModuliRemaining (:350)
-  int32_t ubits = (int32_t)((P.nu + 1) * 32);    // :351
-  int32_t vbits = (int32_t)((P.nv + 1) * 32);    // :352
+  double a[B][K], c[B][K], s[B][K];
+  uint32_t active[B];
+  int32_t status[B], remaining[B], ubits[B], vbits[B];
+  uint32_t ntrace[B], ndig[B];
+  bool live[B];  // still in the elimination loop
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const bool here = (uint32_t)b < npairs;
+    const PairDesc P = args.pairs[Gd.first + (here ? b : 0)];
+    if (here) {
+      residues<T, K>(args.limbs + P.u_off, P.nu, a[b], M, rows, tile);
+      residues<T, K>(args.limbs + P.v_off, P.nv, c[b], M, rows, tile);
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k) a[b][k] = c[b][k] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) s[b][k] = 1.0;
+    active[b] = here ? valid : 0u;
+    status[b] = here ? KST_OK : KST_NOTRUN;
+    remaining[b] = (int32_t)nq;                  // totalModuliRemaining (:350)
+    ubits[b] = (int32_t)((P.nu + 1) * 32);       // :351
+    vbits[b] = (int32_t)((P.nv + 1) * 32);       // :352
+    ntrace[b] = ndig[b] = 0;
+    live[b] = here;
+  }
 
   // ---- (b) elimination ----------------------------------------------------
-  uint64_t lmin = NONE;
+  // Initial keys t = u/v (infinity where v == 0).
+  uint64_t key[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) key[b] = NONE;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if (k < rows && ((valid >> k) & 1)) {
-      int32_t tv = INT32_MIN;
-      if (c[k] != 0.0) tv = to_signed(A::mulmod(a[k], A::modinv(canon(c[k], M[k]), M[k]), M[k]), M[k]);
-      lmin = min64(lmin, elim_key(tv, base + (uint32_t)k * T + t));
+      const uint32_t idx = base + (uint32_t)k * T + t;
+      double y[B], inv[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) y[b] = (live[b] && c[b][k] != 0.0) ? c[b][k] : 1.0;
+      batch_inverse<B, A>(y, inv, M[k]);
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        if (!live[b]) continue;
+        int32_t tv = INT32_MIN;
+        if (c[b][k] != 0.0) tv = to_signed(A::mulmod(a[b][k], inv[b], M[k]), M[k]);
+        key[b] = min64(key[b], elim_key(tv, idx));
+      }
     }
   }
-  uint64_t piv = wg_min<T>(lmin, C);
-  if (C.G > 1) piv = grid_min(piv, C);
+  wg_min<T, B>(key, C);
+  if (C.G > 1) grid_min<B>(key, C);
+  uint64_t piv[B];
+  bool any_live = false;
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    piv[b] = key[b];
+    if (live[b] && piv[b] == NONE) { status[b] = KST_TIMEOUT; live[b] = false; }
+    any_live |= live[b];
+  }
 
-  bool done = (piv == NONE);
-  if (done) status = KST_TIMEOUT;
-  while (!done) {
-    uint32_t pidx;
-    int32_t pval;
-    elim_decode(piv, pidx, pval);
-    if (leader && ntrace < P.trace_cap) {
-      mgcd_pair_t tp;
-      tp.modulus = args.mod_u32[pidx];
-      tp.value = pval;
-      args.trace[P.trace_off + ntrace] = tp;
+  while (any_live) {
+    uint32_t pidx[B];
+    int32_t pval[B];
+    double pm[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      pidx[b] = IDX_MAX + 1;  // matches no modulus
+      pval[b] = 0;
+      pm[b] = 0.0;
+      if (!live[b]) continue;
+      elim_decode(piv[b], pidx[b], pval[b]);
+      const PairDesc& P = args.pairs[Gd.first + b];
+      if (leader && ntrace[b] < P.trace_cap) {
+        mgcd_pair_t tp;
+        tp.modulus = args.mod_u32[pidx[b]];
+        tp.value = pval[b];
+        args.trace[P.trace_off + ntrace[b]] = tp;
+      }
+      ntrace[b]++;
+      pm[b] = args.mod_d[pidx[b]];
+      key[b] = NONE;
     }
-    ntrace++;
-    const double pm = args.mod_d[pidx];
-    const double B = (double)pval;
-    lmin = NONE;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (k < rows && ((valid >> k) & 1)) {
         const uint32_t idx = base + (uint32_t)k * T + t;
-        if (idx == pidx) active &= ~(1u << k);
-        int32_t tv = INT32_MIN;
-        if ((active >> k) & 1) {
-          const double Pp = pm - M[k].m;  // p mod m, signed (p, m in [2^(L-1), 2^L))
-          const double c2 = reduce(a[k] - A::mulmod(B, c[k], M[k]), M[k]);
-          const double a2 = A::mulmod(c[k], Pp, M[k]);
-          s[k] = A::mulmod(s[k], Pp, M[k]);
-          a[k] = a2;
-          c[k] = c2;
-          if (c2 != 0.0) tv = to_signed(A::mulmod(a2, A::modinv(canon(c2, M[k]), M[k]), M[k]), M[k]);
+        // Forward: new state in place, prefix products of the values to
+        // invert (1 where there is nothing to invert).
+        double p[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          double y = 1.0;
+          if (live[b]) {
+            if (idx == pidx[b]) active[b] &= ~(1u << k);
+            if ((active[b] >> k) & 1) {
+              const double Pp = pm[b] - M[k].m;  // p mod m, signed (p, m in [2^(L-1), 2^L))
+              const double c2 = reduce(a[b][k] - A::mulmod((double)pval[b], c[b][k], M[k]), M[k]);
+              a[b][k] = A::mulmod(c[b][k], Pp, M[k]);
+              s[b][k] = A::mulmod(s[b][k], Pp, M[k]);
+              c[b][k] = c2;
+              if (c2 != 0.0) y = c2;
+            }
+          }
+          p[b] = (b == 0) ? y : A::mulmod(p[b - 1], y, M[k]);
         }
-        lmin = min64(lmin, elim_key(tv, idx));
+        // One inverse of the product, then back down: inv_b = r * p_{b-1},
+        // r <- r * y_b (Montgomery's batch inversion).
+        double r = A::modinv(canon(p[B - 1], M[k]), M[k]);
+#pragma unroll
+        for (int b = B - 1; b >= 0; --b) {
+          if (!live[b]) continue;
+          const bool has = ((active[b] >> k) & 1) && c[b][k] != 0.0;
+          int32_t tv = INT32_MIN;
+          if (has) {
+            const double inv = (b == 0) ? r : A::mulmod(r, p[b - 1], M[k]);
+            tv = to_signed(A::mulmod(a[b][k], inv, M[k]), M[k]);
+            if (b > 0) r = A::mulmod(r, c[b][k], M[k]);
+          }
+          key[b] = min64(key[b], elim_key(tv, idx));
+        }
       }
     }
-    uint64_t next = wg_min<T>(lmin, C);
-    if (C.G > 1) next = grid_min(next, C);
-    remaining -= 1;                                        // :390
-    const int32_t tbits = ubits - (L - 1) + ffs_abs(pval);  // :391
-    ubits = vbits;
-    vbits = tbits;
-    if (remaining * (L - 2) <= ubits) { status = KST_REDUX; done = true; break; }  // :393-399
-    if (next == NONE) { status = KST_TIMEOUT; done = true; break; }
-    piv = next;
-    if ((piv & 1) && (uint32_t)(piv >> 20) == 0x80000000u) break;  // pivot value is infinity
+    wg_min<T, B>(key, C);
+    if (C.G > 1) grid_min<B>(key, C);
+    any_live = false;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      if (!live[b]) continue;
+      remaining[b] -= 1;                                             // :390
+      const int32_t tbits = ubits[b] - (L - 1) + ffs_abs(pval[b]);  // :391
+      ubits[b] = vbits[b];
+      vbits[b] = tbits;
+      if (remaining[b] * (L - 2) <= ubits[b]) {                     // :393-399
+        status[b] = KST_REDUX;
+        live[b] = false;
+      } else if (key[b] == NONE) {
+        status[b] = KST_TIMEOUT;
+        live[b] = false;
+      } else {
+        piv[b] = key[b];
+        if ((piv[b] & 1) && (uint32_t)(piv[b] >> 20) == 0x80000000u) live[b] = false;  // infinity: done
+      }
+      any_live |= live[b];
+    }
   }
 
-  // ---- (c) recovery ---------------------------------------------------------
-  if (!done) {
-    // Candidate of this lane: the lowest index holding a nonzero residue.
+  // ---- (c) recovery, pair by pair ------------------------------------------
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    if (status[b] != KST_OK) continue;
+    const PairDesc& P = args.pairs[Gd.first + b];
+    // Candidate: the lowest index holding a nonzero residue (:57-116).
     auto select = [&](void) -> uint64_t {
       uint32_t best = NONE_IDX;
 #pragma unroll
       for (int k = K - 1; k >= 0; --k)
-        if (k < rows && ((active >> k) & 1) && a[k] != 0.0) best = base + (uint32_t)k * T + t;
-      const uint64_t wbest = wg_min<T>((uint64_t)best, C);
-      uint64_t key = (uint64_t)NONE_IDX << 32;
-      if (wbest != NONE_IDX) {
-        if (best == wbest) {
+        if (k < rows && ((active[b] >> k) & 1) && a[b][k] != 0.0) best = base + (uint32_t)k * T + t;
+      uint64_t w[1] = {(uint64_t)best};
+      wg_min<T, 1>(w, C);
+      uint64_t sel[1] = {(uint64_t)NONE_IDX << 32};
+      if (w[0] != NONE_IDX) {
+        if (best == w[0]) {
 #pragma unroll
           for (int k = 0; k < K; ++k) {
             if (base + (uint32_t)k * T + t == best) {
-              const int32_t v = to_signed(A::mulmod(a[k], A::modinv(canon(s[k], M[k]), M[k]), M[k]), M[k]);
-              *sel_lds = (wbest << 32) | (uint32_t)v;
+              const int32_t v =
+                  to_signed(A::mulmod(a[b][k], A::modinv(canon(s[b][k], M[k]), M[k]), M[k]), M[k]);
+              *sel_lds = (w[0] << 32) | (uint32_t)v;
             }
           }
         }
         __syncthreads();
-        key = *sel_lds;
+        sel[0] = uniform64(*sel_lds);
         __syncthreads();
       }
-      if (C.G > 1) key = grid_min(key, C);
-      return key;
+      if (C.G > 1) grid_min<1>(sel, C);
+      return sel[0];
     };
 
     uint64_t sel = select();
-    if (sel == NONE) status = KST_TIMEOUT;
+    if (sel == NONE) status[b] = KST_TIMEOUT;
     while (sel != NONE) {
       uint32_t pidx = (uint32_t)(sel >> 32);
       int32_t pval = (int32_t)(uint32_t)sel;
       if (pidx == NONE_IDX) { pidx = 0; pval = 0; }  // no nonzero residue: thread 0's pair (:57-64)
       const double pm = args.mod_d[pidx];
-      const double Gd = (double)pval;
+      const double gv = (double)pval;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         if (k < rows && ((valid >> k) & 1)) {
           const uint32_t idx = base + (uint32_t)k * T + t;
-          if (idx == pidx) active &= ~(1u << k);
-          if ((active >> k) & 1) {
-            a[k] = reduce(a[k] - A::mulmod(Gd, s[k], M[k]), M[k]);  // u' = (u - g) / p'
-            s[k] = A::mulmod(s[k], pm - M[k].m, M[k]);
+          if (idx == pidx) active[b] &= ~(1u << k);
+          if ((active[b] >> k) & 1) {
+            a[b][k] = reduce(a[b][k] - A::mulmod(gv, s[b][k], M[k]), M[k]);  // u' = (u - g) / p'
+            s[b][k] = A::mulmod(s[b][k], pm - M[k].m, M[k]);
           }
         }
       }
@@ -303,37 +459,42 @@ __device__ void process_item(const KernelArgs& args, const PairDesc& P, uint32_t
         mgcd_pair_t dp;
         dp.modulus = args.mod_u32[pidx];
         dp.value = pval;
-        args.digits[P.dig_off + ndig] = dp;
+        args.digits[P.dig_off + ndig[b]] = dp;
       }
-      ndig++;
-      remaining -= 1;                                                        // :428
-      if (remaining <= 0) { status = pval != 0 ? KST_RECOVERY : KST_OK; break; }  // :429-430, :440
-      if (next == NONE) { status = KST_TIMEOUT; break; }
+      ndig[b]++;
+      remaining[b] -= 1;                                                                // :428
+      if (remaining[b] <= 0) { status[b] = pval != 0 ? KST_RECOVERY : KST_OK; break; }  // :429-430, :440
+      if (next == NONE) { status[b] = KST_TIMEOUT; break; }
       sel = next;
       if ((int32_t)(uint32_t)sel == 0) break;  // value 0: done (:433)
     }
   }
 
   if (leader) {
-    PairOut o;
-    o.status = status;
-    o.ndig = ndig;
-    o.ntrace = ntrace;
-    o.pad = 0;
-    args.out[pair] = o;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      if ((uint32_t)b >= npairs) continue;
+      PairOut o;
+      o.status = status[b];
+      o.ndig = ndig[b];
+      o.ntrace = ntrace[b];
+      o.pad = 0;
+      args.out[Gd.first + b] = o;
+    }
   }
 }
 
-template <int T, int K, class A>
+template <int T, int K, int B, class A>
 __global__ __launch_bounds__(T) void gcd_kernel(KernelArgs args) {
   __shared__ double tile[TILE];
-  __shared__ uint64_t red[2 * (T / 64)];
-  __shared__ uint64_t bc[2];
+  __shared__ uint64_t red[2 * B * (T / 64)];
+  __shared__ uint64_t bc[2 * B];
   __shared__ uint64_t sel_lds;
   __shared__ uint32_t item_lds;
   Ctx C;
   C.red = red;
   C.bc = bc;
+  C.nb = B;
   C.par = 0;
   C.spin_limit = args.spin_limit;
   for (;;) {
@@ -343,12 +504,12 @@ __global__ __launch_bounds__(T) void gcd_kernel(KernelArgs args) {
     __syncthreads();
     if (it >= args.nitems) break;
     const WorkItem W = args.items[it];
-    const PairDesc P = args.pairs[W.pair];
-    C.G = P.G;
+    const GroupDesc Gd = args.groups[W.group];
+    C.G = Gd.G;
     C.g = W.part;
     C.rnd = 0;
-    C.bar = args.bar + P.bar_off;
-    process_item<T, K, A>(args, P, W.pair, tile, C, &sel_lds);
+    C.bar = args.bar + Gd.bar_off;
+    process_group<T, K, B, A>(args, Gd, tile, C, &sel_lds);
   }
 }
 
@@ -420,22 +581,23 @@ __global__ void certify_endpoints_kernel(int L, uint64_t y_lo, uint64_t y_hi, in
 
 // ---------------------------------------------------------------------------
 // Variant table.
-#define MGCD_VARIANT(T, K, A, W) {T, K, W, reinterpret_cast<const void*>(&gcd_kernel<T, K, A>)}
-static const KernelVariant kVariants[] = {
-    MGCD_VARIANT(256, 1, Arith32, 1),  MGCD_VARIANT(256, 2, Arith32, 1),  MGCD_VARIANT(256, 4, Arith32, 1),
-    MGCD_VARIANT(256, 8, Arith32, 1),  MGCD_VARIANT(256, 16, Arith32, 1), MGCD_VARIANT(1024, 1, Arith32, 1),
-    MGCD_VARIANT(1024, 2, Arith32, 1), MGCD_VARIANT(1024, 4, Arith32, 1), MGCD_VARIANT(1024, 8, Arith32, 1),
-    MGCD_VARIANT(256, 1, Arith22, 0),  MGCD_VARIANT(256, 2, Arith22, 0),  MGCD_VARIANT(256, 4, Arith22, 0),
-    MGCD_VARIANT(256, 8, Arith22, 0),  MGCD_VARIANT(256, 16, Arith22, 0), MGCD_VARIANT(1024, 1, Arith22, 0),
-    MGCD_VARIANT(1024, 2, Arith22, 0), MGCD_VARIANT(1024, 4, Arith22, 0), MGCD_VARIANT(1024, 8, Arith22, 0),
-};
+#define MGCD_VARIANT(T, K, B, A, W) {T, K, B, W, reinterpret_cast<const void*>(&gcd_kernel<T, K, B, A>)}
+#define MGCD_VARIANTS(A, W)                                                                              \
+  MGCD_VARIANT(256, 1, 1, A, W), MGCD_VARIANT(256, 2, 1, A, W), MGCD_VARIANT(256, 4, 1, A, W),           \
+      MGCD_VARIANT(256, 8, 1, A, W), MGCD_VARIANT(1024, 1, 1, A, W), MGCD_VARIANT(1024, 2, 1, A, W),     \
+      MGCD_VARIANT(1024, 4, 1, A, W), MGCD_VARIANT(1024, 8, 1, A, W), MGCD_VARIANT(256, 2, 2, A, W),     \
+      MGCD_VARIANT(256, 4, 2, A, W), MGCD_VARIANT(256, 1, 4, A, W), MGCD_VARIANT(256, 2, 4, A, W),       \
+      MGCD_VARIANT(256, 4, 4, A, W), MGCD_VARIANT(256, 1, 8, A, W),      \
+      MGCD_VARIANT(256, 2, 8, A, W)
+static const KernelVariant kVariants[] = {MGCD_VARIANTS(Arith32, 1), MGCD_VARIANTS(Arith22, 0)};
+#undef MGCD_VARIANTS
 #undef MGCD_VARIANT
 
 int kernel_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
 const KernelVariant& kernel_variant(int i) { return kVariants[i]; }
-int find_kernel_variant(int T, int K, int wide) {
+int find_kernel_variant(int T, int K, int B, int wide) {
   for (int i = 0; i < kernel_variant_count(); ++i)
-    if (kVariants[i].T == T && kVariants[i].K == K && kVariants[i].wide == wide) return i;
+    if (kVariants[i].T == T && kVariants[i].K == K && kVariants[i].B == B && kVariants[i].wide == wide) return i;
   return -1;
 }
 

@@ -177,10 +177,12 @@ struct mgcd_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<int> max_resident;  // per kernel variant: co-resident workgroups
   // pooled buffers of the single-pair path
-  DevBuf limbs, pairs, items, queue, bar, digits, trace, out;
+  DevBuf limbs, pairs, groups, items, queue, bar, digits, trace, out;
   double last_kernel_s = 0.0;
   uint32_t force_nq = 0;  // MGCD_OPT_FORCE_MODULI
   bool retry = true;      // MGCD_OPT_RETRY
+  int single_T = 0;       // MGCD_OPT_SINGLE_WORKGROUP (0: automatic)
+  int single_K = 0;       // MGCD_OPT_SINGLE_SLOTS (0: automatic)
 };
 
 // A prepared launch over device-resident pairs.
@@ -195,38 +197,52 @@ struct LaunchSet {
 
 namespace {
 
-// Splits pairs over workgroups of variant `vi`; fills descriptors' G,
-// offsets, and the work-item list (pairs in decreasing order of work so
-// multi-part pairs are popped first).  Returns MGCD_OK or an error.
-int plan_items(mgcd_ctx* ctx, int vi, std::vector<PairDesc>& pd, std::vector<WorkItem>& items, LaunchSet& ls,
-               uint32_t trace_cap) {
+// Plans a launch of variant `vi`: pairs with equal N_Q are grouped, at most
+// B per group; a group is split over G = ceil(N_Q / (T*K)) workgroups.
+// Groups go to the work queue in decreasing order of work (N_Q^2) with the
+// G parts of a group consecutive, so groups spanning several workgroups are
+// popped first.  pd is reordered into group order; order[j] is the caller's
+// index of kernel pair j.
+int plan_items(mgcd_ctx* ctx, int vi, std::vector<PairDesc>& pd, std::vector<uint32_t>& order,
+               std::vector<GroupDesc>& groups, std::vector<WorkItem>& items, LaunchSet& ls, uint32_t trace_cap) {
   const KernelVariant& kv = kernel_variant(vi);
   const uint32_t per_wg = (uint32_t)(kv.T * kv.K);
   const uint32_t maxres = (uint32_t)ctx->max_resident[vi];
   if (maxres == 0) return MGCD_ERR_TOO_MANY_THREADS;
+  order.resize(pd.size());
+  for (uint32_t i = 0; i < pd.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return pd[x].nq > pd[y].nq; });
+  std::vector<PairDesc> sorted;
+  sorted.reserve(pd.size());
+  groups.clear();
   size_t bar = 0, dig = 0, tr = 0;
   uint32_t maxG = 1;
-  for (auto& p : pd) {
-    p.G = (p.nq + per_wg - 1) / per_wg;
-    if (p.G > maxres) return MGCD_ERR_TOO_MANY_THREADS;
-    maxG = std::max(maxG, p.G);
-    p.bar_off = bar;
-    if (p.G > 1) bar += 2 * (size_t)p.G;
+  for (uint32_t j = 0; j < order.size(); ++j) {
+    PairDesc p = pd[order[j]];
     p.dig_off = dig;
     dig += p.nq;
     p.trace_off = tr;
     p.trace_cap = trace_cap;
     tr += trace_cap;
+    if (groups.empty() || groups.back().nq != p.nq || groups.back().count == (uint32_t)kv.B) {
+      GroupDesc gd;
+      std::memset(&gd, 0, sizeof gd);
+      gd.first = j;
+      gd.nq = p.nq;
+      gd.G = (p.nq + per_wg - 1) / per_wg;
+      if (gd.G > maxres) return MGCD_ERR_TOO_MANY_THREADS;
+      maxG = std::max(maxG, gd.G);
+      gd.bar_off = bar;
+      if (gd.G > 1) bar += 2 * (size_t)gd.G * kv.B;
+      groups.push_back(gd);
+    }
+    groups.back().count++;
+    sorted.push_back(p);
   }
-  std::vector<uint32_t> order(pd.size());
-  for (uint32_t i = 0; i < pd.size(); ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-    const uint64_t wx = (uint64_t)pd[x].nq * pd[x].nq, wy = (uint64_t)pd[y].nq * pd[y].nq;
-    return wx > wy;
-  });
+  pd.swap(sorted);
   items.clear();
-  for (uint32_t i : order)
-    for (uint32_t g = 0; g < pd[i].G; ++g) items.push_back(WorkItem{i, g});
+  for (uint32_t gi = 0; gi < groups.size(); ++gi)
+    for (uint32_t g = 0; g < groups[gi].G; ++g) items.push_back(WorkItem{gi, g});
   ls.variant = vi;
   ls.nitems = (uint32_t)items.size();
   ls.grid = std::min<uint32_t>(ls.nitems, maxres);
@@ -253,9 +269,15 @@ int launch(mgcd_ctx* ctx, const LaunchSet& ls, const KernelArgs& ka) {
 
 int choose_single_variant(mgcd_ctx* ctx, uint32_t nq) {
   const int w = ctx->wide;
-  if (nq <= 1024) return find_kernel_variant(1024, 1, w);
-  for (int K : {1, 2, 4, 8, 16}) {
-    const int vi = find_kernel_variant(256, K, w);
+  if (ctx->single_T > 0 && ctx->single_K > 0) {
+    const int vi = find_kernel_variant(ctx->single_T, ctx->single_K, 1, w);
+    if (vi >= 0 && (nq + ctx->single_T * ctx->single_K - 1) / (ctx->single_T * ctx->single_K) <=
+                       (uint32_t)ctx->max_resident[vi])
+      return vi;
+  }
+  if (nq <= 1024) return find_kernel_variant(1024, 1, 1, w);
+  for (int K : {1, 2, 4, 8}) {
+    const int vi = find_kernel_variant(256, K, 1, w);
     if (vi < 0) continue;
     const uint32_t G = (nq + 256 * K - 1) / (256 * K);
     if (G <= (uint32_t)ctx->max_resident[vi]) return vi;
@@ -380,7 +402,7 @@ int mgcd_ctx_destroy(mgcd_ctx* ctx) {
   if (!ctx) return MGCD_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  for (DevBuf* b : {&ctx->d_mod_u32, &ctx->d_mod_d, &ctx->d_mod_inv, &ctx->limbs, &ctx->pairs, &ctx->items,
+  for (DevBuf* b : {&ctx->d_mod_u32, &ctx->d_mod_d, &ctx->d_mod_inv, &ctx->limbs, &ctx->pairs, &ctx->groups, &ctx->items,
                     &ctx->queue, &ctx->bar, &ctx->digits, &ctx->trace, &ctx->out})
     b->release();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -394,7 +416,7 @@ int mgcd_ctx_L(const mgcd_ctx* ctx) { return ctx ? ctx->L : 0; }
 int mgcd_ctx_num_moduli(const mgcd_ctx* ctx) { return ctx ? (int)ctx->moduli.size() : 0; }
 int mgcd_ctx_max_grid(const mgcd_ctx* ctx) {
   if (!ctx) return 0;
-  const int vi = find_kernel_variant(256, 1, ctx->wide);
+  const int vi = find_kernel_variant(256, 1, 1, ctx->wide);
   return vi < 0 ? 0 : ctx->max_resident[vi];
 }
 int mgcd_ctx_moduli(const mgcd_ctx* ctx, uint32_t* out, size_t cap) {
@@ -414,6 +436,12 @@ int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value) {
       return MGCD_OK;
     case MGCD_OPT_RETRY:
       ctx->retry = value != 0;
+      return MGCD_OK;
+    case MGCD_OPT_SINGLE_WORKGROUP:
+      ctx->single_T = (int)value;
+      return MGCD_OK;
+    case MGCD_OPT_SINGLE_SLOTS:
+      ctx->single_K = (int)value;
       return MGCD_OK;
     default:
       return MGCD_ERR_ARG;
@@ -452,12 +480,15 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
   pd[0].nv = (uint32_t)nv;
   pd[0].nq = nq;
   std::vector<WorkItem> items;
+  std::vector<GroupDesc> groups;
+  std::vector<uint32_t> order;
   LaunchSet ls;
   const uint32_t trace_cap = (pivots && tcap) ? (uint32_t)std::min<size_t>(tcap, nq) : 0;
-  int rc = plan_items(ctx, vi, pd, items, ls, trace_cap);
+  int rc = plan_items(ctx, vi, pd, order, groups, items, ls, trace_cap);
   if (rc != MGCD_OK) return rc;
 
   if (ctx->limbs.ensure((nu + nv + 1) * 4) != hipSuccess || ctx->pairs.ensure(sizeof(PairDesc)) != hipSuccess ||
+      ctx->groups.ensure(sizeof(GroupDesc)) != hipSuccess ||
       ctx->items.ensure(items.size() * sizeof(WorkItem)) != hipSuccess || ctx->queue.ensure(4) != hipSuccess ||
       ctx->bar.ensure((ls.nbar + 1) * 8) != hipSuccess || ctx->digits.ensure(ls.ndigits * 8) != hipSuccess ||
       ctx->trace.ensure((ls.ntrace + 1) * 8) != hipSuccess || ctx->out.ensure(sizeof(PairOut)) != hipSuccess)
@@ -466,11 +497,13 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
   if (nu) HIP_TRY(hipMemcpyAsync(ctx->limbs.p, u, nu * 4, hipMemcpyHostToDevice, s));
   if (nv) HIP_TRY(hipMemcpyAsync(ctx->limbs.as<uint32_t>() + nu, v, nv * 4, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(ctx->pairs.p, pd.data(), sizeof(PairDesc), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(ctx->groups.p, groups.data(), sizeof(GroupDesc), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(ctx->items.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
 
   KernelArgs ka;
   ka.limbs = ctx->limbs.as<uint32_t>();
   ka.pairs = ctx->pairs.as<PairDesc>();
+  ka.groups = ctx->groups.as<GroupDesc>();
   ka.items = ctx->items.as<WorkItem>();
   ka.nitems = ls.nitems;
   ka.L = ctx->L;
@@ -573,7 +606,8 @@ struct mgcd_batch {
   std::vector<uint32_t> nv_limbs;    // capacity hint per pair
   std::vector<WorkItem> items;
   LaunchSet ls;
-  DevBuf limbs, pairs, items_d, queue, bar, digits, trace, out;
+  std::vector<GroupDesc> groups;
+  DevBuf limbs, pairs, groups_d, items_d, queue, bar, digits, trace, out;
   std::vector<PairOut> h_out;
   std::vector<mgcd_pair_t> h_digits;
   double device_s = 0.0, host_s = 0.0;
@@ -585,7 +619,7 @@ extern "C" {
 
 int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, const uint32_t* nu,
                       const uint64_t* v_off, const uint32_t* nv, size_t npairs, int opts_workgroup, int opts_slots,
-                      mgcd_batch** out) {
+                      int opts_group, mgcd_batch** out) {
   if (!ctx || !out || (npairs && (!limbs || !u_off || !nu || !v_off || !nv))) return MGCD_ERR_ARG;
   *out = nullptr;
   HIP_TRY(hipSetDevice(ctx->device));
@@ -629,31 +663,34 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
     b->pd.push_back(p);
   }
   // Tiling: explicit, or a default that fits the median pair in one workgroup.
-  int T = opts_workgroup, K = opts_slots;
-  if (T <= 0 || K <= 0) {
-    std::vector<uint32_t> nqs;
-    for (auto& p : b->pd) nqs.push_back(p.nq);
-    uint32_t med = 256;
-    if (!nqs.empty()) {
-      std::nth_element(nqs.begin(), nqs.begin() + nqs.size() / 2, nqs.end());
-      med = nqs[nqs.size() / 2];
-    }
+  int T = opts_workgroup, K = opts_slots, Bp = opts_group;
+  if (T <= 0 || K <= 0 || Bp <= 0) {
     T = 256;
-    K = 1;
-    while (K < 16 && (uint32_t)(T * K) < med) K *= 2;
+    K = 2;
+    Bp = 4;
   }
-  const int vi = find_kernel_variant(T, K, ctx->wide);
+  const int vi = find_kernel_variant(T, K, Bp, ctx->wide);
   int rc = vi < 0 ? MGCD_ERR_ARG : MGCD_OK;
-  if (rc == MGCD_OK && !b->pd.empty()) rc = plan_items(ctx, vi, b->pd, b->items, b->ls, 0);
+  if (rc == MGCD_OK && !b->pd.empty()) {
+    std::vector<uint32_t> order;
+    rc = plan_items(ctx, vi, b->pd, order, b->groups, b->items, b->ls, 0);
+    std::vector<int64_t> pos(order.size());
+    for (size_t j = 0; j < order.size(); ++j) pos[order[j]] = (int64_t)j;
+    for (auto& k : b->kidx)
+      if (k >= 0) k = pos[k];
+  }
   if (rc == MGCD_OK && !b->pd.empty()) {
     const size_t np = b->pd.size();
     if (b->limbs.ensure(packed.size() * 4 + 4) != hipSuccess || b->pairs.ensure(np * sizeof(PairDesc)) != hipSuccess ||
+        b->groups_d.ensure(b->groups.size() * sizeof(GroupDesc)) != hipSuccess ||
         b->items_d.ensure(b->items.size() * sizeof(WorkItem)) != hipSuccess || b->queue.ensure(4) != hipSuccess ||
         b->bar.ensure((b->ls.nbar + 1) * 8) != hipSuccess || b->digits.ensure(b->ls.ndigits * 8) != hipSuccess ||
         b->trace.ensure(8) != hipSuccess || b->out.ensure(np * sizeof(PairOut)) != hipSuccess) {
       rc = MGCD_ERR_NOMEM;
     } else if (hipMemcpy(b->limbs.p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
                hipMemcpy(b->pairs.p, b->pd.data(), np * sizeof(PairDesc), hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(b->groups_d.p, b->groups.data(), b->groups.size() * sizeof(GroupDesc),
+                         hipMemcpyHostToDevice) != hipSuccess ||
                hipMemcpy(b->items_d.p, b->items.data(), b->items.size() * sizeof(WorkItem), hipMemcpyHostToDevice) !=
                    hipSuccess) {
       rc = MGCD_ERR_HIP;
@@ -678,6 +715,7 @@ int mgcd_batch_run(mgcd_batch* b) {
     KernelArgs ka;
     ka.limbs = b->limbs.as<uint32_t>();
     ka.pairs = b->pairs.as<PairDesc>();
+    ka.groups = b->groups_d.as<GroupDesc>();
     ka.items = b->items_d.as<WorkItem>();
     ka.nitems = b->ls.nitems;
     ka.L = ctx->L;
@@ -795,16 +833,13 @@ int mgcd_batch_pair_stats(const mgcd_batch* b, size_t i, uint32_t* nq, uint32_t*
   return MGCD_OK;
 }
 
-int mgcd_batch_tiling(const mgcd_batch* b, int* workgroup, int* slots) {
+int mgcd_batch_tiling(const mgcd_batch* b, int* workgroup, int* slots, int* group) {
   if (!b) return MGCD_ERR_ARG;
-  if (b->ls.variant < 0) {
-    if (workgroup) *workgroup = 0;
-    if (slots) *slots = 0;
-    return MGCD_OK;
-  }
-  const KernelVariant& kv = kernel_variant(b->ls.variant);
-  if (workgroup) *workgroup = kv.T;
-  if (slots) *slots = kv.K;
+  const bool none = b->ls.variant < 0;
+  const KernelVariant* kv = none ? nullptr : &kernel_variant(b->ls.variant);
+  if (workgroup) *workgroup = none ? 0 : kv->T;
+  if (slots) *slots = none ? 0 : kv->K;
+  if (group) *group = none ? 0 : kv->B;
   return MGCD_OK;
 }
 
@@ -812,7 +847,8 @@ int mgcd_batch_destroy(mgcd_batch* b) {
   if (!b) return MGCD_OK;
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
-  for (DevBuf* d : {&b->limbs, &b->pairs, &b->items_d, &b->queue, &b->bar, &b->digits, &b->trace, &b->out})
+  for (DevBuf* d : {&b->limbs, &b->pairs, &b->groups_d, &b->items_d, &b->queue, &b->bar, &b->digits, &b->trace,
+                    &b->out})
     d->release();
   delete b;
   return MGCD_OK;

@@ -41,6 +41,8 @@ EXPORTS = [
 
 OPT_FORCE_MODULI = 1
 OPT_RETRY = 2
+OPT_SINGLE_WORKGROUP = 3
+OPT_SINGLE_SLOTS = 4
 
 
 class Pair(ctypes.Structure):
@@ -84,13 +86,14 @@ def lib():
                                            ctypes.c_size_t, PSZ, PPair, ctypes.c_size_t, PSZ, PPair,
                                            ctypes.c_size_t, PSZ]
         L.mgcd_batch_create.argtypes = [ctypes.c_void_p, P32, P64, P32, P64, P32, ctypes.c_size_t, ctypes.c_int,
-                                        ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+                                        ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         L.mgcd_batch_run.argtypes = [ctypes.c_void_p]
         L.mgcd_batch_status.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
         L.mgcd_batch_result.argtypes = [ctypes.c_void_p, ctypes.c_size_t, P32, ctypes.c_size_t, PSZ]
         L.mgcd_batch_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double)]
-        L.mgcd_batch_tiling.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        PI = ctypes.POINTER(ctypes.c_int)
+        L.mgcd_batch_tiling.argtypes = [ctypes.c_void_p, PI, PI, PI]
         L.mgcd_batch_destroy.argtypes = [ctypes.c_void_p]
         L.mgcd_certify_quasi_quotient.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P64, P64]
         L.mgcd_last_error_detail.restype = ctypes.c_char_p
@@ -238,7 +241,7 @@ def pack_pairs(pairs):
 class Batch:
     """Pairs resident in HBM, computed by the persistent kernel."""
 
-    def __init__(self, ctx, limbs, u_off, nu, v_off, nv, workgroup=0, slots=0):
+    def __init__(self, ctx, limbs, u_off, nu, v_off, nv, workgroup=0, slots=0, group=0):
         self.ctx = ctx
         self.n = len(nu)
         self._keep = [np.ascontiguousarray(x) for x in (limbs, u_off, nu, v_off, nv)]
@@ -247,7 +250,7 @@ class Batch:
         self.nu = nu
         h = ctypes.c_void_p()
         check(lib().mgcd_batch_create(ctx.h, _p32(limbs), u_off.ctypes.data_as(P64), _p32(nu),
-                                      v_off.ctypes.data_as(P64), _p32(nv), self.n, workgroup, slots,
+                                      v_off.ctypes.data_as(P64), _p32(nv), self.n, workgroup, slots, group,
                                       ctypes.byref(h)), "mgcd_batch_create")
         self.h = h
 
@@ -282,9 +285,9 @@ class Batch:
         return q.value, r.value, k.value
 
     def tiling(self):
-        t, k = ctypes.c_int(), ctypes.c_int()
-        check(lib().mgcd_batch_tiling(self.h, ctypes.byref(t), ctypes.byref(k)))
-        return t.value, k.value
+        t, k, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().mgcd_batch_tiling(self.h, ctypes.byref(t), ctypes.byref(k), ctypes.byref(g)))
+        return t.value, k.value, g.value
 
     def close(self):
         if self.h:

@@ -139,12 +139,28 @@ def test_batch_alone_vs_mixed(ctx_for, mgcd, wl):
     assert alone.result(0) == together.result(10) == math.gcd(*probe)
 
 
-@pytest.mark.parametrize("T,K", [(256, 1), (256, 4), (1024, 2)])
-def test_batch_tilings_agree(ctx_for, mgcd, wl, T, K):
+@pytest.mark.parametrize("T,K,B", [(256, 1, 1), (256, 4, 1), (1024, 2, 1), (256, 2, 2), (256, 1, 4),
+                                   (256, 2, 4), (256, 1, 8)])
+def test_batch_tilings_agree(ctx_for, mgcd, wl, T, K, B):
+    # Groups of B pairs with equal N_Q share one modular inverse per modulus;
+    # 27 pairs leave a partial group; the 700-bit pairs form their own groups.
     ctx = ctx_for(32)
-    pairs = wl.headline(24, seed=1, nbits=12000, gbits=1200)
-    b = mgcd.Batch.from_pairs(ctx, pairs, workgroup=T, slots=K)
-    assert b.tiling() == (T, K)
+    pairs = wl.headline(24, seed=1, nbits=12000, gbits=1200) + wl.headline(3, seed=2, nbits=700, gbits=100)
+    b = mgcd.Batch.from_pairs(ctx, pairs, workgroup=T, slots=K, group=B)
+    assert b.tiling() == (T, K, B)
     b.run()
     for i, (u, v) in enumerate(pairs):
         assert b.result(i) == math.gcd(u, v)
+
+
+def test_batch_group_traces_match_single(ctx_for, mgcd, wl):
+    # Every pair of a grouped batch follows its own pivot sequence: the
+    # iteration and digit counts equal those of the pair run alone.
+    ctx = ctx_for(32)
+    pairs = wl.headline(8, seed=3, nbits=9000, gbits=2000)
+    b = mgcd.Batch.from_pairs(ctx, pairs, workgroup=256, slots=2, group=4)
+    b.run()
+    for i, (u, v) in enumerate(pairs):
+        st, g, d, p = ctx.gcd_trace(max(u, v), min(u, v))
+        nq, R, k = b.pair_stats(i)
+        assert (R, k) == (len(p), len(d)) and b.result(i) == g
