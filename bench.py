@@ -28,12 +28,19 @@ import __graft_entry__ as ge  # noqa: E402
 # (= the FP64 vector peak 78.6 TFLOP/s counting an FMA as two flops).
 VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 
-# Algorithmic VALU lane-ops per unit of work ("Roofline").
+# Algorithmic work per unit, in VALU issue slots of this build's gfx950 ISA
+# (a quarter-rate v_rcp_f32 counts 4); see DESIGN.md, "Roofline".
 OPS_PER_RESIDUE_STEP = 6        # one Horner step per limb per modulus
-OPS_PER_ELIM_FIXED = 60         # 4 mulmod + reduce + key, per modulus-iteration
-OPS_PER_EEA_STEP = 14           # one quasi-quotient extended-Euclid step
-EEA_STEPS_32 = 0.843 * math.log(2**32) + 0.06  # mean Euclid steps, random inputs mod ~2^32
+OPS_PER_VALUE = 66              # per modulus-iteration and pair: state update (3 mulmod, reduce),
+                                # key (mulmod, to_signed, key, min), batch-inverse share (3(B-1)/B mulmod)
+OPS_PER_EEA_STEP = 17.5         # one quasi-quotient extended-Euclid step (14.5 VALU + rcp)
+EEA_STEPS = {32: 18.15, 27: 15.23, 22: 12.32}  # mean Euclid steps, random inputs (CPU emulation, 3e6 each)
 OPS_PER_RECOVERY = 13           # 2 mulmod + reduce per modulus-digit
+
+
+def elim_ops_per_unit(L, B):
+    # one extended Euclid per modulus-iteration is shared by the B pairs of a group
+    return OPS_PER_VALUE + EEA_STEPS[L] * OPS_PER_EEA_STEP / B
 
 
 def log(*a):
@@ -52,6 +59,7 @@ def main():
     ap.add_argument("--per-launch-sample", type=int, default=32)
     ap.add_argument("--workgroup", type=int, default=0)
     ap.add_argument("--slots", type=int, default=0)
+    ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -65,9 +73,9 @@ def main():
     pairs = w.headline(args.pairs, seed=7, nbits=args.bits, gbits=args.gbits)
     log(f"generated {len(pairs)} pairs in {time.time() - t0:.1f}s")
     ctx = m.Context(args.L)
-    batch = m.Batch.from_pairs(ctx, pairs, workgroup=args.workgroup, slots=args.slots)
-    T, K = batch.tiling()
-    log(f"batch resident; tiling T={T} K={K}")
+    batch = m.Batch.from_pairs(ctx, pairs, workgroup=args.workgroup, slots=args.slots, group=args.group)
+    T, K, B = batch.tiling()
+    log(f"batch resident; tiling T={T} K={K} B={B}")
 
     for i in range(args.warmup):
         batch.run()
@@ -99,7 +107,7 @@ def main():
         res_steps += nq * (nu + nv)
         elim_units += R * nq - R * (R - 1) // 2 + nq  # initial keys + iterations
         rec_units += sum(nq - R - j for j in range(k))
-    alg_ops = (res_steps * OPS_PER_RESIDUE_STEP + elim_units * (OPS_PER_ELIM_FIXED + EEA_STEPS_32 * OPS_PER_EEA_STEP)
+    alg_ops = (res_steps * OPS_PER_RESIDUE_STEP + elim_units * elim_ops_per_unit(args.L, B)
                + rec_units * OPS_PER_RECOVERY)
     kern_s = statistics.mean(dev_times)
     achieved = alg_ops / kern_s / 1e12
@@ -157,7 +165,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"batched {args.pairs} x {args.bits}-bit pairs, {args.gbits}-bit shared factor, "
                                f"{args.L}-bit moduli", "pairs": args.pairs, "bits": args.bits,
-                   "gcd_bits": args.gbits, "L": args.L, "tiling": {"workgroup": T, "slots": K}},
+                   "gcd_bits": args.gbits, "L": args.L, "tiling": {"workgroup": T, "slots": K, "group": B}},
         "input_bits_per_s": round(value * 2 * args.bits, 1),
         "one_per_launch": {"gcds_per_s": round(single_rate, 3), "p50_ms": round(p50 * 1e3, 3),
                            "p99_ms": round(p99 * 1e3, 3), "sample": args.per_launch_sample},

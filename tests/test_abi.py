@@ -1,0 +1,135 @@
+"""The boundary without a GPU: the C ABI library loads and exports every
+entry point include/mgcd.h declares, the drop-in C++ library exports the
+GmpCudaDevice interface of the reference header, the moduli tables are the
+reference's, the messages are the reference's, and the product refuses to
+run without a device instead of falling back to the CPU."""
+import ctypes
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "modgcd-onegpu_amd")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "mgcd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mgcd_\w+)\s*\(", text)))
+
+
+def test_every_declared_entry_point_is_exported(mgcd):
+    lib = mgcd.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(mgcd.EXPORTS) == names
+
+
+def test_dropin_library_exports_reference_interface():
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(PKG, "lib", "libgmpcuda.so")],
+                         capture_output=True, text=True, check=True).stdout
+    for sym in ("_ZN7GmpCuda13GmpCudaDevice9setDeviceEi",             # static void setDevice(int)
+                "_ZN7GmpCuda13GmpCudaDeviceC1Ev",                     # GmpCudaDevice()
+                "_ZN7GmpCuda13GmpCudaDeviceD1Ev",                     # ~GmpCudaDevice()
+                "_ZN7GmpCuda13GmpCudaDevice3gcdEP12__mpz_structS2_S2_",  # gcd(mpz_t, mpz_t, mpz_t)
+                "_ZN7GmpCuda6moduliE"):                               # extern const uint32_t moduli[]
+        assert sym in out, sym
+
+
+def test_header_is_host_only_and_compiles_reference_style_code(tmp_path):
+    src = tmp_path / "user.cpp"
+    src.write_text(
+        '#include "GmpCuda.h"\n'
+        "int main() {\n"
+        "  GmpCuda::GmpCudaDevice::setDevice(0);\n"
+        "  GmpCuda::GmpCudaDevice dev;\n"
+        "  mpz_t g, u, v; mpz_inits(g, u, v, NULL);\n"
+        "  dev.gcd(g, u, v);\n"
+        "  int n = dev.getMaxGridSize() + GmpCuda::GmpCudaDevice::GCD_BLOCK_SZ + GmpCuda::L + GmpCuda::W;\n"
+        "  GmpCuda::GmpCudaDevice::pair_t p{GmpCuda::moduli[0], 1};\n"
+        "  return n + (int)p.modulus == 0;\n"
+        "}\n")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"),
+                        "-I/opt/conda/include", str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    header = open(os.path.join(ROOT, "include", "GmpCuda.h")).read()
+    assert not re.search(r"#\s*include\s*<hip/|\bhip[A-Z]\w*_t\b", header)
+
+
+def test_messages_are_the_references(mgcd):
+    L = mgcd.lib()
+    want = {1: "Invalid device number.", 2: "Not enough moduli available for given input.",
+            3: "Cannot allocate enough threads to support computation.",
+            4: "Ran out of moduli in the reduction loop.", 5: "Ran out of moduli in the recovery loop.",
+            6: "Unknown error in the gcd kernel."}
+    for code, msg in want.items():
+        assert L.mgcd_strerror(code).decode() == msg
+
+
+def read_table(L):
+    raw = open(os.path.join(PKG, "data", f"moduli_L{L}.bin"), "rb").read()
+    assert raw[:8] == b"MGCDMODS"
+    version, width, W, count = np.frombuffer(raw[8:24], np.uint32)
+    checksum = int(np.frombuffer(raw[24:32], np.uint64)[0])
+    mods = np.frombuffer(raw[48:], np.uint32)
+    assert (version, width, W) == (1, L, 64) and count == len(mods)
+    return mods, checksum
+
+
+def fnv1a64(arr):
+    h = 0xcbf29ce484222325
+    for b in np.asarray(arr, np.uint32).tobytes():
+        h = ((h ^ b) * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def test_table_L32_is_the_references():
+    mods, _ = read_table(32)
+    fx = json.load(open(os.path.join(GOLDEN, "ref_moduli32.json")))
+    assert len(mods) >= fx["count"]
+    assert mods[:64].tolist() == fx["head"]
+    for i, m in fx["samples"]:
+        assert int(mods[i]) == m
+    assert fnv1a64(mods[: fx["count"]]) == int(fx["fnv1a64"], 16)
+
+
+@pytest.mark.parametrize("L,count", [(22, 97358), (27, 262144), (32, 262144)])
+def test_tables_match_oracle_and_checksums(oracle, L, count):
+    mods, checksum = read_table(L)
+    assert len(mods) == count
+    assert fnv1a64(mods) == checksum
+    head = 20000 if L != 22 else count
+    assert np.array_equal(mods[:head], oracle.create_moduli(L, head))
+    assert np.all(mods[:-1] > mods[1:])
+
+
+def test_reference_createModuli_matches_table_when_built():
+    exe = os.path.join(ROOT, "oracle", "_ref", "createModuli")
+    if not os.path.exists(exe):
+        pytest.skip("reference createModuli not built here (oracle/_ref)")
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    ref = [int(x) for x in re.findall(r"^\s+(\d+),", out, re.M)]
+    mods, _ = read_table(32)
+    assert len(ref) == 65536 and mods[:65536].tolist() == ref
+
+
+def test_no_gpu_means_an_error_not_a_fallback(mgcd):
+    n = ctypes.c_int(0)
+    if mgcd.lib().mgcd_device_count(ctypes.byref(n)) == mgcd.OK and n.value > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(mgcd.MgcdError):
+        mgcd.Context(32)
+
+
+def test_sizing_matches_reference_formula_edges(mgcd):
+    # 1-bit operands: the reference divides by ln(1); the library sizes them
+    # as 2-bit operands when it runs them, and the raw formula is unchanged.
+    assert mgcd.num_moduli_needed(2, 32) == 256
+    assert mgcd.num_moduli_needed(1000000, 22) == 92160
