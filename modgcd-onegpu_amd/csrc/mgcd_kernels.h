@@ -18,8 +18,9 @@ struct PairDesc {
   uint32_t pad0;
   uint64_t dig_off;    // first digit slot in the digit buffer
   uint64_t trace_off;  // first pivot-trace slot
-  uint32_t trace_cap;  // 0: no trace
+  uint32_t trace_cap;  // pivot-trace slots (N_Q + 1: every pivot is recorded)
   uint32_t pad;
+  uint64_t res_off;    // residue scratch: u mod m_i at [res_off + i], v at [res_off + N_Q + i]
 };
 
 // Pairs that share N_Q and are processed together by the same workgroups:
@@ -63,7 +64,8 @@ struct KernelArgs {
   const double* mod_inv;       // fl(1 / m)
   unsigned long long* bar;     // barrier slots (zeroed before each launch)
   mgcd_pair_t* digits;
-  mgcd_pair_t* trace;
+  mgcd_pair_t* trace;          // pivots, written by part 0, replayed by every part
+  uint32_t* res;               // residue scratch (2 N_Q per pair)
   PairOut* out;
   uint32_t spin_limit;         // polls before a barrier wait gives up
 };

@@ -1,35 +1,44 @@
 // mgcd_kernels.hip -- gfx950 kernels of the modular GCD (Weber's algorithm).
 //
-// One persistent kernel computes whole GCDs.  A work queue hands each
-// workgroup a work item = (pair, part): the pair's N_Q moduli are split into
-// G contiguous parts, one per workgroup, and inside a workgroup thread t,
-// slot k owns modulus index base + k*T + t (one modulus per lane per slot,
-// slots unrolled into registers).  For each item the workgroup runs
+// One persistent kernel computes whole GCDs.  Pairs with the same number of
+// moduli N_Q form groups of up to B pairs; a group's N_Q moduli are split
+// into G contiguous parts, one per workgroup, and inside a workgroup thread
+// t, slot k owns modulus index base + k*T + t for every pair of the group
+// (one modulus per lane per slot, slots unrolled into registers).  A work
+// queue hands out (group, part) items.  For each item the workgroup runs
 //   (a) residues: limbs staged through LDS in tiles, read from HBM
 //       coalesced, folded by a Horner step per limb and slot
-//       (modMP, getGcdKernel.cu:297-324);
-//   (b) Weber elimination (getGcdKernel.cu:363-402) with the pivot chosen as
-//       the minimum key |t|<<32 | (m - [t>=0]) over all moduli (:143-220);
-//   (c) signed mixed-radix recovery (:404-433), digit = the nonzero residue
-//       of the lowest modulus index (:57-116).
+//       (modMP, getGcdKernel.cu:297-324); kept in HBM for (c);
+//   (b) Weber elimination (getGcdKernel.cu:363-402), the B pairs in
+//       lockstep, the pivot being the minimum key |t|, modulus, sign over
+//       all moduli (:143-220);
+//   (c) replay of the pivot sequence on the surviving moduli, then signed
+//       mixed-radix recovery (:404-433), digit = the nonzero residue of the
+//       lowest modulus index (:57-116), the B pairs in lockstep.
 //
 // Elimination state.  The reference keeps canonical uq, vq and computes
 // tq = (uq - b vq)/p' and t = vq/tq: two modular inverses per modulus per
-// iteration.  Here each lane keeps (a, c, s) with uq = a/s, vq = c/s (mod m):
-//   c' = a - b c,  a' = c p',  s' = s p',   t = a'/c'
-// which are the same residues and the same t (s cancels), so the pivot
-// sequence is identical, with one inverse per modulus per iteration.  At
-// the end uq = a/s and recovery keeps u = n/d the same way: a lane only
-// inverts when it holds the selected digit.
+// iteration.  The pivot choice only needs t = uq/vq, which obeys
+//   t' = p' / (t - b)        (1/0 = infinity, 1/infinity = 0)
+// as long as (uq, vq) != (0, 0) mod m (the map (u, v) -> (v, (u - b v)/p')
+// is invertible, so that state never changes).  Each lane therefore keeps
+// only t per (pair, slot), with sentinels TINF (vq == 0), TZERO (uq == vq
+// == 0, never a pivot) and TRET (retired: was a pivot); one inverse per
+// modulus and iteration remains, and Montgomery's trick shares it between
+// the B pairs of the group, which hold the same modulus in each lane.
+// uq, vq themselves are needed only on the moduli that survive the
+// elimination; (c) recomputes them there from the stored residues by
+// replaying the recorded pivots on (a, c, s) with uq = a/s, vq = c/s:
+//   c' = a - b c,  a' = c p',  s' = s p'.
 //
-// Cross-workgroup reduction (G > 1).  Each workgroup posts its 52-bit key
-// with a 12-bit round tag into its slot of a two-row ring (agent-scope
-// relaxed atomics, one 8-byte store per workgroup per round); wave 0 of
-// every workgroup polls all G slots of the row until every tag matches and
-// takes the minimum.  Rows alternate by round: a workgroup can be at most
-// one round ahead of any other, so the row it overwrites has been read by
-// everyone.  Waits are bounded by spin_limit polls and then report a
-// timeout instead of hanging.
+// Cross-workgroup reduction (G > 1).  Wave 0 of each workgroup posts its B
+// per-pair minima with a 12-bit round tag into its slots of a two-row ring
+// (agent-scope relaxed atomics, one 8-byte store per pair per workgroup
+// per round), then polls all G*B slots of the row until every tag matches.
+// Rows alternate by round: a workgroup can be at most one round ahead of
+// any other, so the row it overwrites has been read by everyone.  Waits
+// are bounded by spin_limit polls and then report a timeout instead of
+// hanging.
 #include "mgcd_arith.h"
 #include "mgcd_kernels.h"
 
@@ -38,24 +47,54 @@ using namespace dev;
 
 namespace {
 
-constexpr int TILE = 1024;                      // limbs per LDS tile
+constexpr int TILE = 1024;                      // doubles of LDS staging (limbs, or 512 pivots)
 constexpr uint32_t IDX_MAX = (1u << 19) - 1;    // moduli indices < 2^19
-constexpr uint64_t PAYLOAD = (1ull << 52) - 1;  // key bits below the tag
-constexpr uint64_t NONE = ~0ull;                // "no candidate" / timeout
+constexpr uint64_t PAYLOAD = (1ull << 52) - 1;  // slot bits below the tag
 constexpr uint32_t NONE_IDX = 0xFFFFFu;
+constexpr uint64_t EXP52 = 0x4330000000000000ull;  // bit pattern of 2^52
 
-// Elimination key: |t| (32 bits) | (IDX_MAX - idx) (19) | [t < 0] (1).  For
-// moduli in descending index order this orders exactly like the
-// reference's (|t| << 32) | (m - [t >= 0]) (getGcdKernel.cu:154).  The
-// infinity value INT32_MIN gives |t| = 2^31, above every finite |t|.
-__device__ __forceinline__ uint64_t elim_key(int32_t t, uint32_t idx) {
-  const uint32_t a = t < 0 ? 0u - (uint32_t)t : (uint32_t)t;
-  return ((uint64_t)a << 20) | ((uint64_t)(IDX_MAX - idx) << 1) | (uint64_t)(t < 0);
+// t sentinels.  Finite t are the reference's toSigned values, |t| < 2^31.
+constexpr double TINF = 2147483648.0;    // vq == 0: the reference's MOD_INFINITY
+constexpr double TZERO = -2147483648.0;  // uq == vq == 0
+constexpr double TRET = 4294967296.0;    // retired modulus
+constexpr double TFIN = 2147483648.0;    // |t| < TFIN: finite
+constexpr double ZOK = 0x1p40;           // within an update: z = (t - b) + ZOK marks "t' = p'/(t - b)"
+
+// Keys are doubles holding exact integers, compared with v_min_f64:
+//   elimination  |t| * 2^20 + 2 (IDX_MAX - idx) + [t < 0]
+//   recovery     idx * 2^32 + (value + 2^31)
+// For moduli in descending index order the elimination key orders exactly
+// like the reference's (|t| << 32) | (m - [t >= 0]) (getGcdKernel.cu:154);
+// |t| = 2^31 (TINF, TZERO) is its infinity.  KINF caps the keys of
+// retired lanes so that every key fits the 52-bit slot payload.
+constexpr double KINF = 2251799814733823.0;  // 2^51 + 2^20 - 1
+constexpr double KNONE = __builtin_huge_val();  // no result (a wait timed out)
+
+// |t - 2^-21| * 2^20 is |t| * 2^20 - 1/2 for t > 0 and |t| * 2^20 + 1/2
+// for t <= 0 (exact: t is an integer below 2^32); ck = 1/2 + 2 (IDX_MAX -
+// idx) completes the key.  t = 0 gets sign bit 1, which decodes to 0 too.
+__device__ __forceinline__ double elim_key(double t, double ck) {
+  return __builtin_fma(__builtin_fabs(t - 0x1p-21), 0x1p20, ck);
+}
+__device__ __forceinline__ uint64_t key_bits(double k) {  // exact integer of a key < 2^52
+  return __builtin_bit_cast(uint64_t, k + 0x1p52) & PAYLOAD;
+}
+__device__ __forceinline__ double key_from_bits(uint64_t x) {
+  return __builtin_bit_cast(double, x | EXP52) - 0x1p52;
 }
 __device__ __forceinline__ void elim_decode(uint64_t key, uint32_t& idx, int32_t& val) {
   const uint32_t a = (uint32_t)(key >> 20);
   idx = IDX_MAX - (uint32_t)((key >> 1) & IDX_MAX);
-  val = (key & 1) ? (int32_t)(0u - a) : (int32_t)a;
+  val = (key & 1) ? (int32_t)(0u - a) : (int32_t)a;  // |t| = 2^31 decodes to INT32_MIN
+}
+
+// The reference's toSigned (getGcdKernel.cu:278-281) of an exact integer
+// r, |r| < 2^40: the representative in [-(m+1)/2, (m-3)/2].  rint((r+1)/m)
+// has no ties (m is odd) and lies 1/(2m) away from them, far above the
+// rounding error.
+__device__ __forceinline__ double tsigned(double r, ModD M) {
+  const double q = __builtin_rint(__builtin_fma(r, M.minv, M.minv));
+  return __builtin_fma(-q, M.m, r);
 }
 
 // __ffs(abs(x)) as at getGcdKernel.cu:391 (abs(INT32_MIN) stays INT32_MIN).
@@ -64,119 +103,165 @@ __device__ __forceinline__ int ffs_abs(int32_t x) {
   return a ? __builtin_ctz(a) + 1 : 0;
 }
 
-__device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return b < a ? b : a; }
-
-// A value known to be the same in every lane, moved to scalar registers so
-// that everything derived from it (pivot index, modulus, bookkeeping) stays
-// scalar too.
-__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
-  return ((uint64_t)hi << 32) | lo;
+// Reads x[i] after `after` has been computed: the empty asm makes the
+// address depend on it, so the compiler cannot hoist the load (and keep
+// the value in registers across a long stretch of code).
+template <class V>
+__device__ __forceinline__ V load_after(const V* x, int i, double after) {
+  uint32_t off = (uint32_t)i * sizeof(V);
+  asm volatile("" : "+v"(off) : "v"(after));
+  return *(const V*)((const char*)x + off);
 }
 
-// Wave-wide minimum, result uniform.  Within each row of 16 lanes DPP
-// quad_perm [1,0,3,2], [2,3,0,1] and row_ror 4, 8 leave the row minimum in
-// every lane; the four rows are then combined through readlane in SGPRs.
-// Must be called with all 64 lanes active.
-__device__ __forceinline__ uint32_t dpp_min_u32(uint32_t x) {
-  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0xB1, 0xF, 0xF, false));
-  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x4E, 0xF, 0xF, false));
-  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x124, 0xF, 0xF, false));
-  x = min(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x128, 0xF, 0xF, false));
-  const uint32_t r0 = __builtin_amdgcn_readlane(x, 0), r1 = __builtin_amdgcn_readlane(x, 16);
-  const uint32_t r2 = __builtin_amdgcn_readlane(x, 32), r3 = __builtin_amdgcn_readlane(x, 48);
-  return min(min(r0, r1), min(r2, r3));
+// ---------------------------------------------------------------------------
+// Cross-lane minima of doubles.
+__device__ __forceinline__ uint32_t lo32(double x) { return (uint32_t)__builtin_bit_cast(uint64_t, x); }
+__device__ __forceinline__ uint32_t hi32(double x) { return (uint32_t)(__builtin_bit_cast(uint64_t, x) >> 32); }
+__device__ __forceinline__ double mk64(uint32_t lo, uint32_t hi) {
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp64(double x) {
+  return mk64((uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo32(x), CTRL, 0xF, 0xF, false),
+              (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi32(x), CTRL, 0xF, 0xF, false));
+}
+// v_permlane32_swap: the upper 32 lanes of a trade places with the lower
+// 32 lanes of b.  v_permlane16_swap: within each half, the odd row (16
+// lanes) of a with the even row of b.
+template <int S>
+__device__ __forceinline__ void swap64(double& a, double& b) {
+  if constexpr (S == 0) {
+    const auto l = __builtin_amdgcn_permlane32_swap(lo32(a), lo32(b), false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi32(a), hi32(b), false, false);
+    a = mk64(l[0], h[0]);
+    b = mk64(l[1], h[1]);
+  } else {
+    const auto l = __builtin_amdgcn_permlane16_swap(lo32(a), lo32(b), false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(hi32(a), hi32(b), false, false);
+    a = mk64(l[0], h[0]);
+    b = mk64(l[1], h[1]);
+  }
+}
+// Stage S of a 64-lane butterfly; partner lanes differ in lane bit 5 - S:
+// S = 0, 1 permlane32/16 swaps, S = 2 row_mirror, 3 row_half_mirror,
+// 4 quad_perm [2,3,0,1], 5 quad_perm [1,0,3,2].
+template <int S>
+constexpr int stage_dpp() {
+  return S == 2 ? 0x140 : S == 3 ? 0x141 : S == 4 ? 0x4E : 0xB1;
+}
+// Halving step: a lane whose bit is 0 keeps the minimum of the first
+// halves (a) of itself and its partner, a lane whose bit is 1 of the
+// second halves (b); the result is left in a.
+template <int S>
+__device__ __forceinline__ void half_step(double& a, double& b) {
+  if constexpr (S <= 1) {
+    swap64<S>(a, b);
+    a = __builtin_fmin(a, b);
+  } else {
+    const bool hi = (__lane_id() >> (5 - S)) & 1;
+    const double send = hi ? a : b;
+    const double keep = hi ? b : a;
+    a = __builtin_fmin(keep, dpp64<stage_dpp<S>()>(send));
+  }
+}
+template <int S>
+__device__ __forceinline__ void full_step(double& x) {
+  if constexpr (S <= 1) {
+    double y = x;
+    swap64<S>(x, y);
+    x = __builtin_fmin(x, y);
+  } else {
+    x = __builtin_fmin(x, dpp64<stage_dpp<S>()>(x));
+  }
+}
+template <int S, int N, int B>
+__device__ __forceinline__ void scatter_min(double (&v)[B]) {
+  if constexpr (S < 6) {
+    if constexpr (N >= 2) {
+#pragma unroll
+      for (int j = 0; j < N / 2; ++j) half_step<S>(v[j], v[j + N / 2]);
+      scatter_min<S + 1, N / 2, B>(v);
+    } else {
+      full_step<S>(v[0]);
+      scatter_min<S + 1, 1, B>(v);
+    }
+  }
+}
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+// Wave minima of B values per lane (B a power of two <= 64), as a
+// reduce-scatter: about B/2 + 6 exchanges instead of 6 per value.
+// Afterwards lane l holds the minimum of value l >> (6 - log2 B).
+template <int B>
+__device__ __forceinline__ double wave_min_scatter(double (&v)[B]) {
+  scatter_min<0, B, B>(v);
+  return v[0];
+}
+// Minimum over the lanes l with the same l % B (B a power of two <= 32).
+template <int B>
+__device__ __forceinline__ double class_min(double x) {
+  if constexpr (B <= 1) x = __builtin_fmin(x, dpp64<0x121>(x));  // row_ror 1
+  if constexpr (B <= 2) x = __builtin_fmin(x, dpp64<0x122>(x));  // row_ror 2
+  if constexpr (B <= 4) x = __builtin_fmin(x, dpp64<0x124>(x));  // row_ror 4
+  if constexpr (B <= 8) x = __builtin_fmin(x, dpp64<0x128>(x));  // row_ror 8
+  if constexpr (B <= 16) full_step<1>(x);
+  full_step<0>(x);
+  return x;
 }
 
-__device__ __forceinline__ uint64_t wave_min(uint64_t x) {
-  const uint32_t hi = (uint32_t)(x >> 32);
-  const uint32_t mh = dpp_min_u32(hi);
-  const uint32_t ml = dpp_min_u32(hi == mh ? (uint32_t)x : 0xFFFFFFFFu);
-  return ((uint64_t)mh << 32) | ml;
-}
+// ---------------------------------------------------------------------------
+// Workgroup state.
+template <int T, int K, int B>
+struct Shared {
+  static constexpr int NW = T / 64;
+  double tile[TILE];          // limb tile / pivot tile (value, modulus)
+  double red[B][NW];          // per-wave minima
+  double bb[B], pp[B];        // current pivot of each pair: value, modulus
+  uint32_t pidx[B];           // its modulus index (NONE_IDX: none)
+  uint32_t flag;              // some pair still running
+  uint64_t actw[B][K][NW];    // surviving moduli: ballot words
+  int32_t status[B];
+  uint32_t ntrace[B];
+  double sel[B];              // recovery: the owner's (index, value) key
+  uint16_t surv[T * K];       // survivors of one pair (local offsets)
+  uint32_t item;
+};
 
 struct Ctx {
-  uint64_t* red;             // LDS [2][B][NW]
-  uint64_t* bc;              // LDS [2][B]
-  int nb;                    // B: per-parity stride of red (B*NW) and bc (B)
-  int par;                   // LDS double-buffer parity
   uint32_t G, g;             // parts of the group, this part
   uint32_t rnd;              // cross-workgroup round counter
   unsigned long long* bar;   // this group's 2*G*B slots
   uint32_t spin_limit;
 };
 
-// Minima of x[0..N) over the workgroup; every thread gets them.  One
-// barrier; the LDS buffer alternates so the next call cannot overwrite
-// values still being read.
-template <int T, int N>
-__device__ __forceinline__ void wg_min(uint64_t (&x)[N], Ctx& C) {
-  constexpr int NW = T / 64;
-  uint64_t* red = C.red + C.par * C.nb * NW;  // fixed stride: calls with other N never overlap
-#pragma unroll
-  for (int b = 0; b < N; ++b) {
-    const uint64_t w = wave_min(x[b]);
-    if ((threadIdx.x & 63) == 0) red[b * NW + (threadIdx.x >> 6)] = w;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int b = 0; b < N; ++b) {
-    uint64_t r = red[b * NW];
-#pragma unroll
-    for (int w = 1; w < NW; ++w) r = min64(r, red[b * NW + w]);
-    x[b] = uniform64(r);
-  }
-  C.par ^= 1;
-}
-
-// Minima of x[0..N) over the G workgroups of the group (x: this workgroup's
-// minima, uniform).  Slot layout of a row: [G][N].  Sets every x[b] to NONE
-// on timeout.
-template <int N>
-__device__ void grid_min(uint64_t (&x)[N], Ctx& C) {
+// Cross-workgroup minima, called by wave 0 (all 64 lanes): lane b < B
+// brings this workgroup's value for pair b and gets the group's minimum;
+// KNONE for every pair when a wait timed out.  Slot layout of a row:
+// [G][B], so lane l polls only slots of pair l % B.
+template <int B>
+__device__ double grid_min(double x, Ctx& C) {
+  const uint32_t lane = __lane_id();
   C.rnd += 1;
   const uint64_t tag = (uint64_t)((C.rnd - 1) % 4095u + 1u);
-  unsigned long long* row = C.bar + (size_t)(C.rnd & 1) * C.G * N;
-  if (threadIdx.x < N) {
-    uint64_t v = x[0];
-#pragma unroll
-    for (int b = 1; b < N; ++b)
-      if ((int)threadIdx.x == b) v = x[b];
-    __hip_atomic_store(row + (size_t)C.g * N + threadIdx.x, (unsigned long long)((tag << 52) | (v & PAYLOAD)),
+  unsigned long long* row = C.bar + (size_t)(C.rnd & 1) * C.G * B;
+  if (lane < (uint32_t)B)
+    __hip_atomic_store(row + (size_t)C.g * B + lane, (unsigned long long)((tag << 52) | key_bits(x)),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  if (threadIdx.x < 64) {
-    uint64_t m[N];
-#pragma unroll
-    for (int b = 0; b < N; ++b) m[b] = NONE;
-    bool late = false;
-    const uint32_t nslots = C.G * N;
-    for (uint32_t j = threadIdx.x; j < nslots; j += 64) {
-      uint64_t v;
-      uint32_t spins = 0;
-      for (;;) {
-        v = __hip_atomic_load(row + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((v >> 52) == tag) break;
-        if (++spins > C.spin_limit) { late = true; break; }
-        __builtin_amdgcn_s_sleep(1);
-      }
-      const uint32_t b = j % N;
-#pragma unroll
-      for (int q = 0; q < N; ++q)
-        if ((uint32_t)q == b) m[q] = min64(m[q], v & PAYLOAD);
+  double m = KNONE;
+  bool late = false;
+  const uint32_t nslots = C.G * B;
+  for (uint32_t j = lane; j < nslots; j += 64) {
+    uint64_t v;
+    uint32_t spins = 0;
+    for (;;) {
+      v = __hip_atomic_load(row + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((v >> 52) == tag) break;
+      if (++spins > C.spin_limit) { late = true; break; }
+      __builtin_amdgcn_s_sleep(1);
     }
-    const bool any_late = __ballot(late) != 0;
-#pragma unroll
-    for (int b = 0; b < N; ++b) {
-      const uint64_t r = wave_min(m[b]);
-      if (threadIdx.x == 0) C.bc[C.par * C.nb + b] = any_late ? NONE : r;
-    }
+    m = __builtin_fmin(m, key_from_bits(v & PAYLOAD));
   }
-  __syncthreads();
-#pragma unroll
-  for (int b = 0; b < N; ++b) x[b] = uniform64(C.bc[C.par * C.nb + b]);
-  C.par ^= 1;
+  m = class_min<B>(m);
+  return __ballot(late) ? KNONE : m;
 }
 
 // (a) Residues of x (n limbs) for every slot row in use.
@@ -200,308 +285,553 @@ __device__ void residues(const uint32_t* __restrict__ x, uint32_t n, double (&r)
   }
 }
 
-// y_b^-1 for y[0..B) (each canonical or signed reduced, nonzero mod m) with
-// one modular inverse: prefix products p_b = y_0 ... y_b, inv = p_{B-1}^-1,
-// then y_b^-1 = inv * p_{b-1} and inv <- inv * y_b going down (Montgomery's
-// batch-inversion trick).  Results signed reduced.
-template <int B, class A>
-__device__ __forceinline__ void batch_inverse(const double (&y)[B], double (&inv)[B], ModD M) {
-  if (B == 1) {
-    inv[0] = A::modinv(canon(y[0], M), M);
-    return;
+// Per-pair bookkeeping, one lane per pair in wave 0 (getGcdKernel.cu:350-352,
+// 389-402 and 417-441).
+struct PairState {
+  int32_t live, status, remaining, ubits, vbits, pval;
+  uint32_t niter, ndig;
+};
+
+// Records pivot (idx, val) as iteration niter of the pair: trace entry
+// (leader workgroup), and the next round's pivot in LDS.
+template <int T, int K, int B>
+__device__ __forceinline__ void publish_pivot(const KernelArgs& args, const PairDesc& P, bool lead, uint32_t b,
+                                              PairState& ps, uint32_t idx, int32_t val, Shared<T, K, B>& S) {
+  if (lead && ps.niter < P.trace_cap) {
+    mgcd_pair_t e;
+    e.modulus = args.mod_u32[idx];
+    e.value = val;
+    args.trace[P.trace_off + ps.niter] = e;
   }
-  double p[B];
-  p[0] = y[0];
-#pragma unroll
-  for (int b = 1; b < B; ++b) p[b] = A::mulmod(p[b - 1], y[b], M);
-  double r = A::modinv(canon(p[B - 1], M), M);
-#pragma unroll
-  for (int b = B - 1; b >= 1; --b) {
-    inv[b] = A::mulmod(r, p[b - 1], M);
-    r = A::mulmod(r, y[b], M);
-  }
-  inv[0] = r;
+  ps.niter++;
+  ps.pval = val;
+  S.bb[b] = (double)val;
+  S.pp[b] = args.mod_d[idx];
+  S.pidx[b] = idx;
 }
 
 // Everything one workgroup does for part C.g of a group of up to B pairs.
-// Pairs of a group run the elimination in lockstep (a pair that has left
-// the loop only skips its arithmetic); recovery then runs pair by pair.
 template <int T, int K, int B, class A>
-__device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, double* tile, Ctx& C,
-                              uint64_t* sel_lds) {
-  const uint32_t t = threadIdx.x;
-  const uint32_t g = C.g;
+__device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Shared<T, K, B>& S, Ctx& C) {
+  constexpr int NW = T / 64;
+  constexpr int SH = 6 - ilog2(B);  // lane l of a reduce-scatter holds pair l >> SH
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63, wave = tid >> 6;
   const uint32_t nq = Gd.nq;
+  const uint32_t npairs = Gd.count;
   // Part g owns moduli [g*nq/G, (g+1)*nq/G): sizes differ by at most one,
   // none is empty (G <= nq), none exceeds T*K (checked by the host).
-  const uint32_t base = (uint32_t)(((uint64_t)g * nq) / C.G);
-  const uint32_t cnt = (uint32_t)(((uint64_t)(g + 1) * nq) / C.G) - base;
+  const uint32_t base = (uint32_t)(((uint64_t)C.g * nq) / C.G);
+  const uint32_t cnt = (uint32_t)(((uint64_t)(C.g + 1) * nq) / C.G) - base;
   const int rows = (int)((cnt + T - 1) / T);
-  const bool leader = (g == 0) && (t == 0);
+  const bool lead = C.g == 0;
   const int L = args.L;
-  const uint32_t npairs = Gd.count;
 
   ModD M[K];
-  uint32_t valid = 0;
+  double ck[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const uint32_t j = (uint32_t)k * T + t;
+    const uint32_t j = (uint32_t)k * T + tid;
     if (j < cnt) {
       M[k].m = args.mod_d[base + j];
       M[k].minv = args.mod_inv[base + j];
-      valid |= 1u << k;
     } else {
-      M[k].m = 3.0;  // unused lane of a partial row: any modulus keeps it harmless
+      M[k].m = 3.0;  // no modulus here: any modulus keeps the lane harmless
       M[k].minv = 1.0 / 3.0;
     }
+    ck[k] = 0.5 + 2.0 * (double)(IDX_MAX - (base + j));
   }
 
-  double a[B][K], c[B][K], s[B][K];
-  uint32_t active[B];
-  int32_t status[B], remaining[B], ubits[B], vbits[B];
-  uint32_t ntrace[B], ndig[B];
-  bool live[B];  // still in the elimination loop
+  // ---- (a) residues of u and v, stored canonical for (c) -----------------
+  for (uint32_t b = 0; b < npairs; ++b) {
+    const PairDesc& P = args.pairs[Gd.first + b];
+    double r[K];
+    residues<T, K>(args.limbs + P.u_off, P.nu, r, M, rows, S.tile);
 #pragma unroll
-  for (int b = 0; b < B; ++b) {
-    const bool here = (uint32_t)b < npairs;
-    const PairDesc P = args.pairs[Gd.first + (here ? b : 0)];
-    if (here) {
-      residues<T, K>(args.limbs + P.u_off, P.nu, a[b], M, rows, tile);
-      residues<T, K>(args.limbs + P.v_off, P.nv, c[b], M, rows, tile);
-    } else {
+    for (int k = 0; k < K; ++k)
+      if ((uint32_t)k * T + tid < cnt) args.res[P.res_off + base + k * T + tid] = (uint32_t)canon(r[k], M[k]);
+    residues<T, K>(args.limbs + P.v_off, P.nv, r, M, rows, S.tile);
 #pragma unroll
-      for (int k = 0; k < K; ++k) a[b][k] = c[b][k] = 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) s[b][k] = 1.0;
-    active[b] = here ? valid : 0u;
-    status[b] = here ? KST_OK : KST_NOTRUN;
-    remaining[b] = (int32_t)nq;                  // totalModuliRemaining (:350)
-    ubits[b] = (int32_t)((P.nu + 1) * 32);       // :351
-    vbits[b] = (int32_t)((P.nv + 1) * 32);       // :352
-    ntrace[b] = ndig[b] = 0;
-    live[b] = here;
+    for (int k = 0; k < K; ++k)
+      if ((uint32_t)k * T + tid < cnt) args.res[P.res_off + nq + base + k * T + tid] = (uint32_t)canon(r[k], M[k]);
   }
 
   // ---- (b) elimination ----------------------------------------------------
-  // Initial keys t = u/v (infinity where v == 0).
-  uint64_t key[B];
-#pragma unroll
-  for (int b = 0; b < B; ++b) key[b] = NONE;
+  // Initial t = u/v (getGcdKernel.cu:369), one batched inverse per slot.
+  double t[B][K];
+  double key[B];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    if (k < rows && ((valid >> k) & 1)) {
-      const uint32_t idx = base + (uint32_t)k * T + t;
-      double y[B], inv[B];
-#pragma unroll
-      for (int b = 0; b < B; ++b) y[b] = (live[b] && c[b][k] != 0.0) ? c[b][k] : 1.0;
-      batch_inverse<B, A>(y, inv, M[k]);
-#pragma unroll
-      for (int b = 0; b < B; ++b) {
-        if (!live[b]) continue;
-        int32_t tv = INT32_MIN;
-        if (c[b][k] != 0.0) tv = to_signed(A::mulmod(a[b][k], inv[b], M[k]), M[k]);
-        key[b] = min64(key[b], elim_key(tv, idx));
-      }
-    }
-  }
-  wg_min<T, B>(key, C);
-  if (C.G > 1) grid_min<B>(key, C);
-  uint64_t piv[B];
-  bool any_live = false;
-#pragma unroll
-  for (int b = 0; b < B; ++b) {
-    piv[b] = key[b];
-    if (live[b] && piv[b] == NONE) { status[b] = KST_TIMEOUT; live[b] = false; }
-    any_live |= live[b];
-  }
-
-  while (any_live) {
-    uint32_t pidx[B];
-    int32_t pval[B];
-    double pm[B];
+    const uint32_t j = (uint32_t)k * T + tid;
+    const bool here = j < cnt;
+    double P[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-      pidx[b] = IDX_MAX + 1;  // matches no modulus
-      pval[b] = 0;
-      pm[b] = 0.0;
-      if (!live[b]) continue;
-      elim_decode(piv[b], pidx[b], pval[b]);
-      const PairDesc& P = args.pairs[Gd.first + b];
-      if (leader && ntrace[b] < P.trace_cap) {
-        mgcd_pair_t tp;
-        tp.modulus = args.mod_u32[pidx[b]];
-        tp.value = pval[b];
-        args.trace[P.trace_off + ntrace[b]] = tp;
-      }
-      ntrace[b]++;
-      pm[b] = args.mod_d[pidx[b]];
-      key[b] = NONE;
+      double v = 0.0;
+      if (here && (uint32_t)b < npairs) v = (double)args.res[args.pairs[Gd.first + b].res_off + nq + base + j];
+      const double y = v != 0.0 ? v : 1.0;
+      P[b] = b == 0 ? y : A::mulmod(P[b - 1], y, M[k]);
     }
+    double r = A::modinv(canon(P[B - 1], M[k]), M[k]);
+#pragma unroll
+    for (int b = B - 1; b >= 0; --b) {
+      double u = 0.0, v = 0.0;
+      if (here && (uint32_t)b < npairs) {
+        const uint64_t ro = args.pairs[Gd.first + b].res_off + base + j;
+        u = (double)args.res[ro];
+        v = (double)args.res[ro + nq];
+      }
+      const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
+      if (b > 0) r = A::mulmod(r, v != 0.0 ? v : 1.0, M[k]);
+      double tv = v != 0.0 ? tsigned(A::mulmod(u, inv, M[k]), M[k]) : (u != 0.0 ? TINF : TZERO);
+      if (!here) tv = TRET;
+      t[b][k] = tv;
+      const double kk = elim_key(tv, ck[k]);
+      key[b] = k == 0 ? kk : __builtin_fmin(key[b], kk);
+    }
+  }
+
+  // Per-pair state lives in lane b of wave 0.
+  PairState ps;
+  ps.live = 0;
+  ps.status = KST_NOTRUN;
+  ps.remaining = ps.ubits = ps.vbits = ps.pval = 0;
+  ps.niter = ps.ndig = 0;
+  if (wave == 0 && lane < npairs) {
+    const PairDesc& Pl = args.pairs[Gd.first + lane];
+    ps.live = 1;
+    ps.status = KST_OK;
+    ps.remaining = (int32_t)nq;                   // totalModuliRemaining (:350)
+    ps.ubits = (int32_t)((Pl.nu + 1) * 32);       // :351
+    ps.vbits = (int32_t)((Pl.nv + 1) * 32);       // :352
+  }
+
+  // Reduces key[] over the group; wave 0 lane b < B gets pair b's minimum.
+  auto reduce_keys = [&]() -> double {
+    const double w = wave_min_scatter<B>(key);
+    if ((lane & ((1u << SH) - 1)) == 0) S.red[lane >> SH][wave] = w;
+    __syncthreads();
+    double x = KNONE;
+    if (wave == 0) {
+      if (lane < (uint32_t)B) {
+        x = S.red[lane][0];
+#pragma unroll
+        for (int w2 = 1; w2 < NW; ++w2) x = __builtin_fmin(x, S.red[lane][w2]);
+        x = __builtin_fmin(x, KINF);
+      }
+      if (C.G > 1) x = grid_min<B>(x, C);
+    }
+    return x;
+  };
+
+  // First pivot: the loop body runs once even for an infinite one (do-while, :374-402).
+  {
+    const double kr = reduce_keys();
+    if (wave == 0) {
+      if (lane < npairs) {
+        if (kr == KNONE) {
+          ps.status = KST_TIMEOUT;
+          ps.live = 0;
+        } else {
+          uint32_t idx;
+          int32_t val;
+          elim_decode(key_bits(kr), idx, val);
+          publish_pivot(args, args.pairs[Gd.first + lane], lead, lane, ps, idx, val, S);
+        }
+      }
+      if (lane < (uint32_t)B && !ps.live) {
+        S.bb[lane] = 0.0;
+        S.pp[lane] = 1.0;
+        S.pidx[lane] = NONE_IDX;
+      }
+      const bool any = __ballot(ps.live != 0) != 0;
+      if (lane == 0) S.flag = any;
+    }
+    __syncthreads();
+  }
+
+  while (S.flag) {
+    // Retire each pair's pivot modulus (:376-377).
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const uint32_t off = __builtin_amdgcn_readfirstlane(S.pidx[b]) - base;
+      if (off < cnt) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (off == (uint32_t)k * T + tid) t[b][k] = TRET;
+      }
+    }
+    // t' = p' / (t - b) for every pair (:379-385): prefix products of the
+    // values to invert (1 where there is nothing to invert), one inverse,
+    // then back down (Montgomery's batch inversion).
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      if (k < rows && ((valid >> k) & 1)) {
-        const uint32_t idx = base + (uint32_t)k * T + t;
-        // Forward: new state in place, prefix products of the values to
-        // invert (1 where there is nothing to invert).
-        double p[B];
+      if (k >= rows) break;
+      // Forward: the value to invert is y = t - b where that is invertible
+      // (|y| < 2^33), else 1.  t is replaced by z = y + ZOK there, and by its
+      // final value elsewhere: t - b == 0 gives infinity, infinity gives 0,
+      // retired and zero states stay.
+      double P[B];
 #pragma unroll
-        for (int b = 0; b < B; ++b) {
-          double y = 1.0;
-          if (live[b]) {
-            if (idx == pidx[b]) active[b] &= ~(1u << k);
-            if ((active[b] >> k) & 1) {
-              const double Pp = pm[b] - M[k].m;  // p mod m, signed (p, m in [2^(L-1), 2^L))
-              const double c2 = reduce(a[b][k] - A::mulmod((double)pval[b], c[b][k], M[k]), M[k]);
-              a[b][k] = A::mulmod(c[b][k], Pp, M[k]);
-              s[b][k] = A::mulmod(s[b][k], Pp, M[k]);
-              c[b][k] = c2;
-              if (c2 != 0.0) y = c2;
-            }
-          }
-          p[b] = (b == 0) ? y : A::mulmod(p[b - 1], y, M[k]);
-        }
-        // One inverse of the product, then back down: inv_b = r * p_{b-1},
-        // r <- r * y_b (Montgomery's batch inversion).
-        double r = A::modinv(canon(p[B - 1], M[k]), M[k]);
+      for (int b = 0; b < B; ++b) {
+        const double tv = t[b][k];
+        const double d = tv - load_after(S.bb, b, b == 0 ? tv : P[b - 1]);
+        const bool fin = __builtin_fabs(tv) < TFIN;
+        const bool ok = fin && d != 0.0 && __builtin_fabs(d) != M[k].m;
+        t[b][k] = ok ? d + ZOK : (tv == TINF ? 0.0 : (fin ? TINF : tv));
+        P[b] = b == 0 ? (ok ? d : 1.0) : A::mulmod(P[b - 1], ok ? d : 1.0, M[k]);
+      }
+      double r = A::modinv(canon(P[B - 1], M[k]), M[k]);
 #pragma unroll
-        for (int b = B - 1; b >= 0; --b) {
-          if (!live[b]) continue;
-          const bool has = ((active[b] >> k) & 1) && c[b][k] != 0.0;
-          int32_t tv = INT32_MIN;
-          if (has) {
-            const double inv = (b == 0) ? r : A::mulmod(r, p[b - 1], M[k]);
-            tv = to_signed(A::mulmod(a[b][k], inv, M[k]), M[k]);
-            if (b > 0) r = A::mulmod(r, c[b][k], M[k]);
-          }
-          key[b] = min64(key[b], elim_key(tv, idx));
-        }
+      for (int b = B - 1; b >= 0; --b) {
+        const double z = t[b][k];
+        const bool ok = z > 0.5 * ZOK;
+        const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
+        if (b > 0) r = A::mulmod(r, ok ? z - ZOK : 1.0, M[k]);
+        const double tn = tsigned(A::mulmod(load_after(S.pp, b, inv), inv, M[k]), M[k]);
+        const double tv2 = ok ? tn : z;
+        t[b][k] = tv2;
+        const double kk = elim_key(tv2, ck[k]);
+        key[b] = k == 0 ? kk : __builtin_fmin(key[b], kk);
       }
     }
-    wg_min<T, B>(key, C);
-    if (C.G > 1) grid_min<B>(key, C);
-    any_live = false;
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-      if (!live[b]) continue;
-      remaining[b] -= 1;                                             // :390
-      const int32_t tbits = ubits[b] - (L - 1) + ffs_abs(pval[b]);  // :391
-      ubits[b] = vbits[b];
-      vbits[b] = tbits;
-      if (remaining[b] * (L - 2) <= ubits[b]) {                     // :393-399
-        status[b] = KST_REDUX;
-        live[b] = false;
-      } else if (key[b] == NONE) {
-        status[b] = KST_TIMEOUT;
-        live[b] = false;
-      } else {
-        piv[b] = key[b];
-        if ((piv[b] & 1) && (uint32_t)(piv[b] >> 20) == 0x80000000u) live[b] = false;  // infinity: done
+    const double kr = reduce_keys();
+    if (wave == 0) {
+      if (lane < npairs && ps.live) {
+        ps.remaining -= 1;                                          // :390
+        const int32_t tbits = ps.ubits - (L - 1) + ffs_abs(ps.pval);  // :391
+        ps.ubits = ps.vbits;
+        ps.vbits = tbits;
+        if (ps.remaining * (L - 2) <= ps.ubits) {                   // :393-399
+          ps.status = KST_REDUX;
+          ps.live = 0;
+        } else if (kr == KNONE) {
+          ps.status = KST_TIMEOUT;
+          ps.live = 0;
+        } else {
+          uint32_t idx;
+          int32_t val;
+          elim_decode(key_bits(kr), idx, val);
+          if (val == INT32_MIN) ps.live = 0;  // infinity: done (:402)
+          else publish_pivot(args, args.pairs[Gd.first + lane], lead, lane, ps, idx, val, S);
+        }
       }
-      any_live |= live[b];
+      if (lane < (uint32_t)B && !ps.live) {
+        S.bb[lane] = 0.0;
+        S.pp[lane] = 1.0;
+        S.pidx[lane] = NONE_IDX;
+      }
+      const bool any = __ballot(ps.live != 0) != 0;
+      if (lane == 0) S.flag = any;
     }
+    __syncthreads();
   }
 
-  // ---- (c) recovery, pair by pair ------------------------------------------
+  // Surviving moduli of each pair, as ballot words.
 #pragma unroll
-  for (int b = 0; b < B; ++b) {
-    if (status[b] != KST_OK) continue;
+  for (int b = 0; b < B; ++b)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint64_t w = __ballot(t[b][k] != TRET);
+      if (lane == 0) S.actw[b][k][wave] = w;
+    }
+  if (wave == 0 && lane < (uint32_t)B) {
+    S.status[lane] = lane < npairs ? ps.status : KST_NOTRUN;
+    S.ntrace[lane] = ps.niter;
+  }
+  // The pivots recorded by the leader workgroup become visible to all parts.
+  if (C.G > 1 && wave == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    (void)grid_min<B>(0.0, C);
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+  // ---- (c1) replay of the pivots on the survivors, pair by pair ----------
+  for (uint32_t b = 0; b < npairs; ++b) {
+    if (S.status[b] != KST_OK) continue;
     const PairDesc& P = args.pairs[Gd.first + b];
-    // Candidate: the lowest index holding a nonzero residue (:57-116).
-    auto select = [&](void) -> uint64_t {
-      uint32_t best = NONE_IDX;
+    const uint32_t R = S.ntrace[b];
+    // Compaction: survivor n (in slot-major, wave, lane order) -> S.surv[n].
+    uint32_t nsurv = 0;
 #pragma unroll
-      for (int k = K - 1; k >= 0; --k)
-        if (k < rows && ((active[b] >> k) & 1) && a[b][k] != 0.0) best = base + (uint32_t)k * T + t;
-      uint64_t w[1] = {(uint64_t)best};
-      wg_min<T, 1>(w, C);
-      uint64_t sel[1] = {(uint64_t)NONE_IDX << 32};
-      if (w[0] != NONE_IDX) {
-        if (best == w[0]) {
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        const uint64_t word = S.actw[b][k][w];
+        if ((uint32_t)w == wave && ((word >> lane) & 1))
+          S.surv[nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(word >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)word, 0))] =
+              (uint16_t)(k * T + tid);
+        nsurv += (uint32_t)__builtin_popcountll(word);
+      }
+    __syncthreads();
+    ModD RM[K];
+    double ra[K], rc[K], rs[K];
+    uint64_t ro[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t j = (uint32_t)k * T + tid;
+      RM[k].m = 3.0;
+      RM[k].minv = 1.0 / 3.0;
+      ra[k] = rc[k] = 0.0;
+      rs[k] = 1.0;
+      ro[k] = 0;
+      if (j < nsurv) {
+        const uint32_t idx = base + S.surv[j];
+        RM[k].m = args.mod_d[idx];
+        RM[k].minv = args.mod_inv[idx];
+        ro[k] = P.res_off + idx;
+        ra[k] = (double)args.res[ro[k]];
+        rc[k] = (double)args.res[ro[k] + nq];
+      }
+    }
+    // Slots in use by this wave (uniform).
+    int kw = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if ((uint32_t)k * T + wave * 64 < nsurv) kw = k + 1;
+    for (uint32_t c0 = 0; c0 < R; c0 += TILE / 2) {
+      const uint32_t len = min((uint32_t)(TILE / 2), R - c0);
+      __syncthreads();
+      for (uint32_t j = tid; j < len; j += T) {
+        const mgcd_pair_t e = args.trace[P.trace_off + c0 + j];
+        S.tile[2 * j] = (double)e.value;
+        S.tile[2 * j + 1] = (double)e.modulus;
+      }
+      __syncthreads();
+      if (kw > 0) {
+        for (uint32_t j = 0; j < len; ++j) {
+          const double bv = S.tile[2 * j], pv = S.tile[2 * j + 1];
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            if (base + (uint32_t)k * T + t == best) {
-              const int32_t v =
-                  to_signed(A::mulmod(a[b][k], A::modinv(canon(s[b][k], M[k]), M[k]), M[k]), M[k]);
-              *sel_lds = (w[0] << 32) | (uint32_t)v;
+            if (k < kw) {
+              const double c2 = ra[k] - A::mulmod(bv, rc[k], RM[k]);  // |c2| <= m + 2^14
+              ra[k] = A::mulmod(rc[k], pv, RM[k]);
+              rs[k] = A::mulmod(rs[k], pv, RM[k]);
+              rc[k] = c2;
             }
           }
         }
-        __syncthreads();
-        sel[0] = uniform64(*sel_lds);
-        __syncthreads();
       }
-      if (C.G > 1) grid_min<1>(sel, C);
-      return sel[0];
-    };
+    }
+    // u = a/s on the survivors, kept as (a, s) in place of (u, v).
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if ((uint32_t)k * T + tid < nsurv) {
+        args.res[ro[k]] = (uint32_t)canon(ra[k], RM[k]);
+        args.res[ro[k] + nq] = (uint32_t)canon(rs[k], RM[k]);
+      }
+    }
+    __syncthreads();
+  }
 
-    uint64_t sel = select();
-    if (sel == NONE) status[b] = KST_TIMEOUT;
-    while (sel != NONE) {
-      uint32_t pidx = (uint32_t)(sel >> 32);
-      int32_t pval = (int32_t)(uint32_t)sel;
-      if (pidx == NONE_IDX) { pidx = 0; pval = 0; }  // no nonzero residue: thread 0's pair (:57-64)
-      const double pm = args.mod_d[pidx];
-      const double gv = (double)pval;
+  // ---- (c2) recovery, RB pairs at a time in lockstep (:404-433) -----------
+  // u = a/s per surviving modulus; a lane inverts s only for a digit it owns.
+  // Chunk c holds pairs c*RB + j; wave 0 lane c*RB + j keeps pair c*RB + j's
+  // bookkeeping and gets its minima from the slots of class j.
+  constexpr int RB = B < 8 ? B : 8;
+  if (wave == 0 && lane < npairs) ps.live = ps.status == KST_OK;
+  for (uint32_t c0 = 0; c0 < npairs; c0 += RB) {
+    double ua[RB][K], us[RB][K];
+    uint32_t am[RB];  // surviving slots (bits)
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const uint32_t b = c0 + j;
+      am[j] = 0;
+      const bool run = b < npairs && S.status[b] == KST_OK;
+      const uint64_t ro = run ? args.pairs[Gd.first + b].res_off : 0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        if (k < rows && ((valid >> k) & 1)) {
-          const uint32_t idx = base + (uint32_t)k * T + t;
-          if (idx == pidx) active[b] &= ~(1u << k);
-          if ((active[b] >> k) & 1) {
-            a[b][k] = reduce(a[b][k] - A::mulmod(gv, s[b][k], M[k]), M[k]);  // u' = (u - g) / p'
-            s[b][k] = A::mulmod(s[b][k], pm - M[k].m, M[k]);
+        ua[j][k] = 0.0;
+        us[j][k] = 1.0;
+        if (run && ((S.actw[b][k][wave] >> lane) & 1)) {
+          am[j] |= 1u << k;
+          const uint32_t idx = base + (uint32_t)k * T + tid;
+          ua[j][k] = (double)args.res[ro + idx];
+          us[j][k] = (double)args.res[ro + nq + idx];
+        }
+      }
+    }
+    const bool mine = wave == 0 && lane >= c0 && lane < c0 + RB;  // this lane's pair is in the chunk
+
+    // Selects each pair's next digit: the lowest index holding a nonzero u;
+    // wave 0 lane c0 + j gets (index, value) of pair c0 + j as a recovery key.
+    auto select = [&]() -> double {
+      constexpr int SHR = 6 - ilog2(RB);
+      double cand[RB];
+#pragma unroll
+      for (int j = 0; j < RB; ++j) {
+        cand[j] = (double)NONE_IDX;
+#pragma unroll
+        for (int k = K - 1; k >= 0; --k)
+          if (((am[j] >> k) & 1) && ua[j][k] != 0.0) cand[j] = (double)(base + (uint32_t)k * T + tid);
+      }
+      const double w = wave_min_scatter<RB>(cand);
+      if ((lane & ((1u << SHR) - 1)) == 0) S.red[lane >> SHR][wave] = w;
+      __syncthreads();
+      // Owners compute toSigned(u) (:424); one batched inverse per slot.
+      uint32_t own[K];  // bit j: this slot holds pair c0 + j's candidate
+#pragma unroll
+      for (int k = 0; k < K; ++k) own[k] = 0;
+#pragma unroll
+      for (int j = 0; j < RB; ++j) {
+        double x = S.red[j][0];
+#pragma unroll
+        for (int w2 = 1; w2 < NW; ++w2) x = __builtin_fmin(x, S.red[j][w2]);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (((am[j] >> k) & 1) && (double)(base + (uint32_t)k * T + tid) == x) own[k] |= 1u << j;
+        // No nonzero residue in this part: (NONE_IDX, 0), which the reference
+        // resolves to its thread 0 pair with value 0 (:57-64).
+        if ((uint32_t)j == tid && x == (double)NONE_IDX) S.sel[j] = (double)NONE_IDX * 0x1p32 + 0x1p31;
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (__ballot(own[k] != 0) == 0) continue;
+        const double idx = (double)(base + (uint32_t)k * T + tid);
+        double P[RB];
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+          const double y = ((own[k] >> j) & 1) ? us[j][k] : 1.0;
+          P[j] = j == 0 ? y : A::mulmod(P[j - 1], y, M[k]);
+        }
+        double r = A::modinv(canon(P[RB - 1], M[k]), M[k]);
+#pragma unroll
+        for (int j = RB - 1; j >= 0; --j) {
+          const bool owns = (own[k] >> j) & 1;
+          const double inv = j == 0 ? r : A::mulmod(r, P[j - 1], M[k]);
+          if (j > 0) r = A::mulmod(r, owns ? us[j][k] : 1.0, M[k]);
+          if (owns) {
+            const double v = tsigned(A::mulmod(ua[j][k], inv, M[k]), M[k]);
+            S.sel[j] = idx * 0x1p32 + (v + 0x1p31);
           }
         }
       }
-      const uint64_t next = select();
-      if (leader) {
-        mgcd_pair_t dp;
-        dp.modulus = args.mod_u32[pidx];
-        dp.value = pval;
-        args.digits[P.dig_off + ndig[b]] = dp;
+      __syncthreads();
+      double x = KNONE;
+      if (wave == 0) {
+        x = S.sel[lane % RB];
+        if (C.G > 1) x = grid_min<RB>(x, C);
       }
-      ndig[b]++;
-      remaining[b] -= 1;                                                                // :428
-      if (remaining[b] <= 0) { status[b] = pval != 0 ? KST_RECOVERY : KST_OK; break; }  // :429-430, :440
-      if (next == NONE) { status[b] = KST_TIMEOUT; break; }
-      sel = next;
-      if ((int32_t)(uint32_t)sel == 0) break;  // value 0: done (:433)
+      return x;
+    };
+
+    // Publishes digit (idx, val) of pair b (this lane's) as the next update.
+    auto set_digit = [&](uint32_t idx, int32_t val) {
+      S.bb[lane - c0] = (double)val;
+      S.pp[lane - c0] = args.mod_d[idx];
+      S.pidx[lane - c0] = idx;
+      ps.pval = val;
+    };
+    auto decode_sel = [&](double x, uint32_t& idx, int32_t& val) {
+      const uint64_t kb = key_bits(x);
+      idx = (uint32_t)(kb >> 32);
+      val = (int32_t)((uint32_t)kb - 0x80000000u);
+      if (idx == NONE_IDX) {
+        idx = 0;
+        val = 0;
+      }
+    };
+    // Pairs not (or no longer) in recovery publish an update that matches
+    // no modulus.
+    auto finish_round = [&]() {
+      if (wave == 0) {
+        const uint32_t b = c0 + lane % RB;
+        const bool running = __shfl(ps.live, (int)(b & 63)) != 0 && b < npairs;
+        if (lane < (uint32_t)RB) {
+          if (!running) {
+            S.bb[lane] = 0.0;
+            S.pp[lane] = 1.0;
+            S.pidx[lane] = NONE_IDX;
+          }
+        }
+        const bool any = __ballot(mine && ps.live != 0) != 0;
+        if (lane == 0) S.flag = any;
+      }
+      __syncthreads();
+    };
+
+    {
+      const double x = select();
+      if (mine && lane < npairs && ps.live) {
+        if (x == KNONE) {
+          ps.status = KST_TIMEOUT;
+          ps.live = 0;
+        } else {
+          uint32_t idx;
+          int32_t val;
+          decode_sel(x, idx, val);
+          set_digit(idx, val);
+        }
+      }
+      finish_round();
+    }
+    while (S.flag) {
+      // u' = (u - g) / p' on every other surviving modulus (:417-425).
+#pragma unroll
+      for (int j = 0; j < RB; ++j) {
+        const uint32_t pidx = __builtin_amdgcn_readfirstlane(S.pidx[j]);
+        const double gv = S.bb[j], pv = S.pp[j];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if ((am[j] >> k) & 1) {
+            if (base + (uint32_t)k * T + tid == pidx) {
+              am[j] &= ~(1u << k);
+            } else {
+              ua[j][k] = reduce(ua[j][k] - A::mulmod(gv, us[j][k], M[k]), M[k]);
+              us[j][k] = A::mulmod(us[j][k], pv, M[k]);
+            }
+          }
+        }
+      }
+      const double x = select();
+      if (mine && lane < npairs && ps.live) {
+        if (lead) {
+          mgcd_pair_t dp;
+          dp.modulus = args.mod_u32[S.pidx[lane - c0]];
+          dp.value = ps.pval;
+          args.digits[args.pairs[Gd.first + lane].dig_off + ps.ndig] = dp;
+        }
+        ps.ndig++;
+        ps.remaining -= 1;                                       // :428
+        if (ps.remaining <= 0) {                                 // :429-430, :440
+          ps.status = ps.pval != 0 ? KST_RECOVERY : KST_OK;
+          ps.live = 0;
+        } else if (x == KNONE) {
+          ps.status = KST_TIMEOUT;
+          ps.live = 0;
+        } else {
+          uint32_t idx;
+          int32_t val;
+          decode_sel(x, idx, val);
+          if (val == 0) ps.live = 0;  // :433
+          else set_digit(idx, val);
+        }
+      }
+      finish_round();
     }
   }
 
-  if (leader) {
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-      if ((uint32_t)b >= npairs) continue;
-      PairOut o;
-      o.status = status[b];
-      o.ndig = ndig[b];
-      o.ntrace = ntrace[b];
-      o.pad = 0;
-      args.out[Gd.first + b] = o;
-    }
+  if (lead && wave == 0 && lane < npairs) {
+    PairOut o;
+    o.status = ps.status;
+    o.ndig = ps.ndig;
+    o.ntrace = ps.niter;
+    o.pad = 0;
+    args.out[Gd.first + lane] = o;
   }
 }
 
 template <int T, int K, int B, class A>
 __global__ __launch_bounds__(T) void gcd_kernel(KernelArgs args) {
-  __shared__ double tile[TILE];
-  __shared__ uint64_t red[2 * B * (T / 64)];
-  __shared__ uint64_t bc[2 * B];
-  __shared__ uint64_t sel_lds;
-  __shared__ uint32_t item_lds;
+  __shared__ Shared<T, K, B> S;
   Ctx C;
-  C.red = red;
-  C.bc = bc;
-  C.nb = B;
-  C.par = 0;
   C.spin_limit = args.spin_limit;
   for (;;) {
-    if (threadIdx.x == 0) item_lds = atomicAdd(args.queue, 1u);
     __syncthreads();
-    const uint32_t it = item_lds;
+    if (threadIdx.x == 0) S.item = atomicAdd(args.queue, 1u);
     __syncthreads();
+    const uint32_t it = S.item;
     if (it >= args.nitems) break;
     const WorkItem W = args.items[it];
     const GroupDesc Gd = args.groups[W.group];
@@ -509,7 +839,7 @@ __global__ __launch_bounds__(T) void gcd_kernel(KernelArgs args) {
     C.g = W.part;
     C.rnd = 0;
     C.bar = args.bar + Gd.bar_off;
-    process_group<T, K, B, A>(args, Gd, tile, C, &sel_lds);
+    process_group<T, K, B, A>(args, Gd, S, C);
   }
 }
 
@@ -585,10 +915,8 @@ __global__ void certify_endpoints_kernel(int L, uint64_t y_lo, uint64_t y_hi, in
 #define MGCD_VARIANTS(A, W)                                                                              \
   MGCD_VARIANT(256, 1, 1, A, W), MGCD_VARIANT(256, 2, 1, A, W), MGCD_VARIANT(256, 4, 1, A, W),           \
       MGCD_VARIANT(256, 8, 1, A, W), MGCD_VARIANT(1024, 1, 1, A, W), MGCD_VARIANT(1024, 2, 1, A, W),     \
-      MGCD_VARIANT(1024, 4, 1, A, W), MGCD_VARIANT(1024, 8, 1, A, W), MGCD_VARIANT(256, 2, 2, A, W),     \
-      MGCD_VARIANT(256, 4, 2, A, W), MGCD_VARIANT(256, 1, 4, A, W), MGCD_VARIANT(256, 2, 4, A, W),       \
-      MGCD_VARIANT(256, 4, 4, A, W), MGCD_VARIANT(256, 1, 8, A, W),      \
-      MGCD_VARIANT(256, 2, 8, A, W)
+      MGCD_VARIANT(256, 1, 8, A, W), MGCD_VARIANT(256, 2, 8, A, W), MGCD_VARIANT(256, 1, 16, A, W),      \
+      MGCD_VARIANT(256, 2, 16, A, W), MGCD_VARIANT(256, 1, 32, A, W), MGCD_VARIANT(128, 1, 16, A, W)
 static const KernelVariant kVariants[] = {MGCD_VARIANTS(Arith32, 1), MGCD_VARIANTS(Arith22, 0)};
 #undef MGCD_VARIANTS
 #undef MGCD_VARIANT

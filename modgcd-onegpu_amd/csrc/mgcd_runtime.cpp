@@ -177,7 +177,7 @@ struct mgcd_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<int> max_resident;  // per kernel variant: co-resident workgroups
   // pooled buffers of the single-pair path
-  DevBuf limbs, pairs, groups, items, queue, bar, digits, trace, out;
+  DevBuf limbs, pairs, groups, items, queue, bar, digits, trace, res, out;
   double last_kernel_s = 0.0;
   uint32_t force_nq = 0;  // MGCD_OPT_FORCE_MODULI
   bool retry = true;      // MGCD_OPT_RETRY
@@ -193,6 +193,7 @@ struct LaunchSet {
   size_t nbar = 0;       // barrier slots
   size_t ndigits = 0;    // digit slots
   size_t ntrace = 0;     // trace slots
+  size_t nres = 0;       // residue scratch words
 };
 
 namespace {
@@ -204,7 +205,7 @@ namespace {
 // popped first.  pd is reordered into group order; order[j] is the caller's
 // index of kernel pair j.
 int plan_items(mgcd_ctx* ctx, int vi, std::vector<PairDesc>& pd, std::vector<uint32_t>& order,
-               std::vector<GroupDesc>& groups, std::vector<WorkItem>& items, LaunchSet& ls, uint32_t trace_cap) {
+               std::vector<GroupDesc>& groups, std::vector<WorkItem>& items, LaunchSet& ls) {
   const KernelVariant& kv = kernel_variant(vi);
   const uint32_t per_wg = (uint32_t)(kv.T * kv.K);
   const uint32_t maxres = (uint32_t)ctx->max_resident[vi];
@@ -215,15 +216,19 @@ int plan_items(mgcd_ctx* ctx, int vi, std::vector<PairDesc>& pd, std::vector<uin
   std::vector<PairDesc> sorted;
   sorted.reserve(pd.size());
   groups.clear();
-  size_t bar = 0, dig = 0, tr = 0;
+  size_t bar = 0, dig = 0, tr = 0, res = 0;
   uint32_t maxG = 1;
   for (uint32_t j = 0; j < order.size(); ++j) {
     PairDesc p = pd[order[j]];
     p.dig_off = dig;
     dig += p.nq;
+    // Per-pair regions start on 256-byte boundaries, so no cache line is
+    // shared between pairs that different workgroups write and read.
     p.trace_off = tr;
-    p.trace_cap = trace_cap;
-    tr += trace_cap;
+    p.trace_cap = p.nq + 1;
+    tr += (p.trace_cap + 31) & ~(size_t)31;
+    p.res_off = res;
+    res += (2 * (size_t)p.nq + 63) & ~(size_t)63;
     if (groups.empty() || groups.back().nq != p.nq || groups.back().count == (uint32_t)kv.B) {
       GroupDesc gd;
       std::memset(&gd, 0, sizeof gd);
@@ -250,6 +255,7 @@ int plan_items(mgcd_ctx* ctx, int vi, std::vector<PairDesc>& pd, std::vector<uin
   ls.nbar = bar;
   ls.ndigits = dig;
   ls.ntrace = tr;
+  ls.nres = res;
   return MGCD_OK;
 }
 
@@ -403,7 +409,7 @@ int mgcd_ctx_destroy(mgcd_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (DevBuf* b : {&ctx->d_mod_u32, &ctx->d_mod_d, &ctx->d_mod_inv, &ctx->limbs, &ctx->pairs, &ctx->groups, &ctx->items,
-                    &ctx->queue, &ctx->bar, &ctx->digits, &ctx->trace, &ctx->out})
+                    &ctx->queue, &ctx->bar, &ctx->digits, &ctx->trace, &ctx->res, &ctx->out})
     b->release();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -483,15 +489,16 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
   std::vector<GroupDesc> groups;
   std::vector<uint32_t> order;
   LaunchSet ls;
-  const uint32_t trace_cap = (pivots && tcap) ? (uint32_t)std::min<size_t>(tcap, nq) : 0;
-  int rc = plan_items(ctx, vi, pd, order, groups, items, ls, trace_cap);
+  const size_t trace_cap = (pivots && tcap) ? std::min<size_t>(tcap, nq) : 0;
+  int rc = plan_items(ctx, vi, pd, order, groups, items, ls);
   if (rc != MGCD_OK) return rc;
 
   if (ctx->limbs.ensure((nu + nv + 1) * 4) != hipSuccess || ctx->pairs.ensure(sizeof(PairDesc)) != hipSuccess ||
       ctx->groups.ensure(sizeof(GroupDesc)) != hipSuccess ||
       ctx->items.ensure(items.size() * sizeof(WorkItem)) != hipSuccess || ctx->queue.ensure(4) != hipSuccess ||
       ctx->bar.ensure((ls.nbar + 1) * 8) != hipSuccess || ctx->digits.ensure(ls.ndigits * 8) != hipSuccess ||
-      ctx->trace.ensure((ls.ntrace + 1) * 8) != hipSuccess || ctx->out.ensure(sizeof(PairOut)) != hipSuccess)
+      ctx->trace.ensure((ls.ntrace + 1) * 8) != hipSuccess || ctx->res.ensure((ls.nres + 1) * 4) != hipSuccess ||
+      ctx->out.ensure(sizeof(PairOut)) != hipSuccess)
     return MGCD_ERR_NOMEM;
   hipStream_t s = ctx->stream;
   if (nu) HIP_TRY(hipMemcpyAsync(ctx->limbs.p, u, nu * 4, hipMemcpyHostToDevice, s));
@@ -514,6 +521,7 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
   ka.bar = ctx->bar.as<unsigned long long>();
   ka.digits = ctx->digits.as<mgcd_pair_t>();
   ka.trace = ctx->trace.as<mgcd_pair_t>();
+  ka.res = ctx->res.as<uint32_t>();
   ka.out = ctx->out.as<PairOut>();
   ka.spin_limit = kSpinLimit;
   rc = launch(ctx, ls, ka);
@@ -607,7 +615,7 @@ struct mgcd_batch {
   std::vector<WorkItem> items;
   LaunchSet ls;
   std::vector<GroupDesc> groups;
-  DevBuf limbs, pairs, groups_d, items_d, queue, bar, digits, trace, out;
+  DevBuf limbs, pairs, groups_d, items_d, queue, bar, digits, trace, res, out;
   std::vector<PairOut> h_out;
   std::vector<mgcd_pair_t> h_digits;
   double device_s = 0.0, host_s = 0.0;
@@ -666,14 +674,14 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
   int T = opts_workgroup, K = opts_slots, Bp = opts_group;
   if (T <= 0 || K <= 0 || Bp <= 0) {
     T = 256;
-    K = 2;
-    Bp = 4;
+    K = 1;
+    Bp = 16;
   }
   const int vi = find_kernel_variant(T, K, Bp, ctx->wide);
   int rc = vi < 0 ? MGCD_ERR_ARG : MGCD_OK;
   if (rc == MGCD_OK && !b->pd.empty()) {
     std::vector<uint32_t> order;
-    rc = plan_items(ctx, vi, b->pd, order, b->groups, b->items, b->ls, 0);
+    rc = plan_items(ctx, vi, b->pd, order, b->groups, b->items, b->ls);
     std::vector<int64_t> pos(order.size());
     for (size_t j = 0; j < order.size(); ++j) pos[order[j]] = (int64_t)j;
     for (auto& k : b->kidx)
@@ -685,7 +693,8 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
         b->groups_d.ensure(b->groups.size() * sizeof(GroupDesc)) != hipSuccess ||
         b->items_d.ensure(b->items.size() * sizeof(WorkItem)) != hipSuccess || b->queue.ensure(4) != hipSuccess ||
         b->bar.ensure((b->ls.nbar + 1) * 8) != hipSuccess || b->digits.ensure(b->ls.ndigits * 8) != hipSuccess ||
-        b->trace.ensure(8) != hipSuccess || b->out.ensure(np * sizeof(PairOut)) != hipSuccess) {
+        b->trace.ensure(b->ls.ntrace * 8) != hipSuccess || b->res.ensure(b->ls.nres * 4) != hipSuccess ||
+        b->out.ensure(np * sizeof(PairOut)) != hipSuccess) {
       rc = MGCD_ERR_NOMEM;
     } else if (hipMemcpy(b->limbs.p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
                hipMemcpy(b->pairs.p, b->pd.data(), np * sizeof(PairDesc), hipMemcpyHostToDevice) != hipSuccess ||
@@ -726,6 +735,7 @@ int mgcd_batch_run(mgcd_batch* b) {
     ka.bar = b->bar.as<unsigned long long>();
     ka.digits = b->digits.as<mgcd_pair_t>();
     ka.trace = b->trace.as<mgcd_pair_t>();
+    ka.res = b->res.as<uint32_t>();
     ka.out = b->out.as<PairOut>();
     ka.spin_limit = kSpinLimit;
     int rc = launch(ctx, b->ls, ka);
@@ -847,7 +857,7 @@ int mgcd_batch_destroy(mgcd_batch* b) {
   if (!b) return MGCD_OK;
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
-  for (DevBuf* d : {&b->limbs, &b->pairs, &b->groups_d, &b->items_d, &b->queue, &b->bar, &b->digits, &b->trace,
+  for (DevBuf* d : {&b->limbs, &b->pairs, &b->groups_d, &b->items_d, &b->queue, &b->bar, &b->digits, &b->trace, &b->res,
                     &b->out})
     d->release();
   delete b;

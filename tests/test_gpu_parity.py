@@ -139,8 +139,8 @@ def test_batch_alone_vs_mixed(ctx_for, mgcd, wl):
     assert alone.result(0) == together.result(10) == math.gcd(*probe)
 
 
-@pytest.mark.parametrize("T,K,B", [(256, 1, 1), (256, 4, 1), (1024, 2, 1), (256, 2, 2), (256, 1, 4),
-                                   (256, 2, 4), (256, 1, 8)])
+@pytest.mark.parametrize("T,K,B", [(256, 1, 1), (256, 4, 1), (1024, 2, 1), (256, 1, 8), (256, 2, 8),
+                                   (256, 1, 16), (256, 2, 16), (256, 1, 32), (128, 1, 16)])
 def test_batch_tilings_agree(ctx_for, mgcd, wl, T, K, B):
     # Groups of B pairs with equal N_Q share one modular inverse per modulus;
     # 27 pairs leave a partial group; the 700-bit pairs form their own groups.
@@ -158,7 +158,7 @@ def test_batch_group_traces_match_single(ctx_for, mgcd, wl):
     # iteration and digit counts equal those of the pair run alone.
     ctx = ctx_for(32)
     pairs = wl.headline(8, seed=3, nbits=9000, gbits=2000)
-    b = mgcd.Batch.from_pairs(ctx, pairs, workgroup=256, slots=2, group=4)
+    b = mgcd.Batch.from_pairs(ctx, pairs, workgroup=256, slots=1, group=16)
     b.run()
     for i, (u, v) in enumerate(pairs):
         st, g, d, p = ctx.gcd_trace(max(u, v), min(u, v))
