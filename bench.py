@@ -29,13 +29,16 @@ import __graft_entry__ as ge  # noqa: E402
 VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 
 # Algorithmic work per unit, in VALU issue slots of this build's gfx950 ISA
-# (a quarter-rate v_rcp_f32 counts 4); see DESIGN.md, "Roofline".
+# (instruction counts of the compiled loops; a quarter-rate v_rcp_f32 counts
+# 4).  Units are the algorithm's own: active modulus-iterations, not the
+# lanes the kernel keeps busy after their modulus retired.
 OPS_PER_RESIDUE_STEP = 6        # one Horner step per limb per modulus
-OPS_PER_VALUE = 66              # per modulus-iteration and pair: state update (3 mulmod, reduce),
-                                # key (mulmod, to_signed, key, min), batch-inverse share (3(B-1)/B mulmod)
+OPS_PER_VALUE = 61              # per active modulus-iteration and pair: t' = p'/(t - b) with the
+                                # batch-inversion share (4 mulmod), toSigned, key, selects, reduce-scatter
 OPS_PER_EEA_STEP = 17.5         # one quasi-quotient extended-Euclid step (14.5 VALU + rcp)
 EEA_STEPS = {32: 18.15, 27: 15.23, 22: 12.32}  # mean Euclid steps, random inputs (CPU emulation, 3e6 each)
-OPS_PER_RECOVERY = 13           # 2 mulmod + reduce per modulus-digit
+OPS_PER_REPLAY = 19             # per surviving modulus and recorded pivot: 3 mulmod + subtract
+OPS_PER_RECOVERY = 15           # 2 mulmod + reduce per modulus-digit
 
 
 def elim_ops_per_unit(L, B):
@@ -99,16 +102,17 @@ def main():
         assert g > 0 and u % g == 0 and v % g == 0, f"pair {i}: result does not divide the operands"
 
     # Work counts for the roofline.
-    res_steps = elim_units = rec_units = 0
+    res_steps = elim_units = rep_units = rec_units = 0
     for i in range(args.pairs):
         nq, R, k = batch.pair_stats(i)
         nu = (pairs[i][0].bit_length() + 31) // 32
         nv = (pairs[i][1].bit_length() + 31) // 32
         res_steps += nq * (nu + nv)
         elim_units += R * nq - R * (R - 1) // 2 + nq  # initial keys + iterations
+        rep_units += (nq - R) * R                    # survivors x pivots
         rec_units += sum(nq - R - j for j in range(k))
     alg_ops = (res_steps * OPS_PER_RESIDUE_STEP + elim_units * elim_ops_per_unit(args.L, B)
-               + rec_units * OPS_PER_RECOVERY)
+               + rep_units * OPS_PER_REPLAY + rec_units * OPS_PER_RECOVERY)
     kern_s = statistics.mean(dev_times)
     achieved = alg_ops / kern_s / 1e12
 

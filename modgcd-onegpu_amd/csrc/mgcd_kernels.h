@@ -51,14 +51,22 @@ struct PairOut {
 
 enum : int32_t { KST_OK = 0, KST_REDUX = 1, KST_RECOVERY = 2, KST_TIMEOUT = 3, KST_NOTRUN = 4 };
 
+// Work queues: queue 0 is popped first by every workgroup (groups spanning
+// many workgroups); queues 1..NQ-1 belong to the XCDs (all parts of a group
+// on one XCD: the cross-workgroup exchange stays within one L2); a
+// workgroup whose queues are empty takes items from the other XCDs'.
+constexpr int kMaxQueues = 9;
+constexpr int kQueueStride = 16;  // counters 64 bytes apart
+
 struct KernelArgs {
   const uint32_t* limbs;
   const PairDesc* pairs;
   const GroupDesc* groups;
-  const WorkItem* items;
-  uint32_t nitems;
+  const WorkItem* items;       // queue q: items [qoff[q], qoff[q+1])
+  uint32_t nqueues;
+  uint32_t qoff[kMaxQueues + 1];
   int L;
-  uint32_t* queue;             // [0]: next work item
+  uint32_t* queue;             // [q * kQueueStride]: next item of queue q
   const uint32_t* mod_u32;     // moduli table
   const double* mod_d;         // moduli as double
   const double* mod_inv;       // fl(1 / m)

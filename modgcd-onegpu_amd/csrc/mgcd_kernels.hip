@@ -108,9 +108,10 @@ __device__ __forceinline__ int ffs_abs(int32_t x) {
 // the value in registers across a long stretch of code).
 template <class V>
 __device__ __forceinline__ V load_after(const V* x, int i, double after) {
-  uint32_t off = (uint32_t)i * sizeof(V);
-  asm volatile("" : "+v"(off) : "v"(after));
-  return *(const V*)((const char*)x + off);
+  typedef const __attribute__((address_space(3))) V* LdsPtr;
+  LdsPtr p = (LdsPtr)x;
+  asm volatile("" : "+v"(p) : "v"(after));
+  return p[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -223,6 +224,7 @@ struct Shared {
   uint32_t ntrace[B];
   double sel[B];              // recovery: the owner's (index, value) key
   uint16_t surv[T * K];       // survivors of one pair (local offsets)
+  uint32_t retire[T * K];     // bit b: this part's modulus is pair b's pivot
   uint32_t item;
 };
 
@@ -358,13 +360,16 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
 
   // ---- (b) elimination ----------------------------------------------------
   // Initial t = u/v (getGcdKernel.cu:369), one batched inverse per slot.
+  // Keys are reduced over the wave slot by slot (reduce-scatter), so only
+  // one slot's B keys are ever held: wkey in lane l is the wave's minimum
+  // for pair l >> SH.
   double t[B][K];
-  double key[B];
+  double wkey = KNONE;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const uint32_t j = (uint32_t)k * T + tid;
     const bool here = j < cnt;
-    double P[B];
+    double P[B], key[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       double v = 0.0;
@@ -386,9 +391,9 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       double tv = v != 0.0 ? tsigned(A::mulmod(u, inv, M[k]), M[k]) : (u != 0.0 ? TINF : TZERO);
       if (!here) tv = TRET;
       t[b][k] = tv;
-      const double kk = elim_key(tv, ck[k]);
-      key[b] = k == 0 ? kk : __builtin_fmin(key[b], kk);
+      key[b] = elim_key(tv, ck[k]);
     }
+    wkey = __builtin_fmin(wkey, wave_min_scatter<B>(key));
   }
 
   // Per-pair state lives in lane b of wave 0.
@@ -406,9 +411,8 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
     ps.vbits = (int32_t)((Pl.nv + 1) * 32);       // :352
   }
 
-  // Reduces key[] over the group; wave 0 lane b < B gets pair b's minimum.
-  auto reduce_keys = [&]() -> double {
-    const double w = wave_min_scatter<B>(key);
+  // Reduces the wave minima over the group; wave 0 lane b < B gets pair b's.
+  auto reduce_keys = [&](double w) -> double {
     if ((lane & ((1u << SH) - 1)) == 0) S.red[lane >> SH][wave] = w;
     __syncthreads();
     double x = KNONE;
@@ -424,9 +428,17 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
     return x;
   };
 
+#pragma unroll
+  for (int k = 0; k < K; ++k) S.retire[k * T + tid] = 0;
+  // Marks pair b's pivot modulus for retirement if it lies in this part.
+  auto mark_retire = [&](uint32_t b, uint32_t idx) {
+    const uint32_t off = idx - base;
+    if (off < cnt) atomicOr(&S.retire[off], 1u << b);
+  };
+
   // First pivot: the loop body runs once even for an infinite one (do-while, :374-402).
   {
-    const double kr = reduce_keys();
+    const double kr = reduce_keys(wkey);
     if (wave == 0) {
       if (lane < npairs) {
         if (kr == KNONE) {
@@ -437,6 +449,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
           int32_t val;
           elim_decode(key_bits(kr), idx, val);
           publish_pivot(args, args.pairs[Gd.first + lane], lead, lane, ps, idx, val, S);
+          mark_retire(lane, idx);
         }
       }
       if (lane < (uint32_t)B && !ps.live) {
@@ -451,19 +464,21 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   }
 
   while (S.flag) {
-    // Retire each pair's pivot modulus (:376-377).
+    // Retire each pair's pivot modulus (:376-377); few waves hold one.
 #pragma unroll
-    for (int b = 0; b < B; ++b) {
-      const uint32_t off = __builtin_amdgcn_readfirstlane(S.pidx[b]) - base;
-      if (off < cnt) {
+    for (int k = 0; k < K; ++k) {
+      const uint32_t rw = S.retire[k * T + tid];
+      if (__ballot(rw != 0)) {
+        if (rw) S.retire[k * T + tid] = 0;
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-          if (off == (uint32_t)k * T + tid) t[b][k] = TRET;
+        for (int b = 0; b < B; ++b)
+          if ((rw >> b) & 1) t[b][k] = TRET;
       }
     }
     // t' = p' / (t - b) for every pair (:379-385): prefix products of the
     // values to invert (1 where there is nothing to invert), one inverse,
     // then back down (Montgomery's batch inversion).
+    wkey = KNONE;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (k >= rows) break;
@@ -471,31 +486,36 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       // (|y| < 2^33), else 1.  t is replaced by z = y + ZOK there, and by its
       // final value elsewhere: t - b == 0 gives infinity, infinity gives 0,
       // retired and zero states stay.
-      double P[B];
+      double P[B], key[B];
 #pragma unroll
       for (int b = 0; b < B; ++b) {
+        // Branch-free: every candidate is computed, then selected.
         const double tv = t[b][k];
         const double d = tv - load_after(S.bb, b, b == 0 ? tv : P[b - 1]);
         const bool fin = __builtin_fabs(tv) < TFIN;
-        const bool ok = fin && d != 0.0 && __builtin_fabs(d) != M[k].m;
-        t[b][k] = ok ? d + ZOK : (tv == TINF ? 0.0 : (fin ? TINF : tv));
-        P[b] = b == 0 ? (ok ? d : 1.0) : A::mulmod(P[b - 1], ok ? d : 1.0, M[k]);
+        const bool ok = fin & (d != 0.0) & (__builtin_fabs(d) != M[k].m);
+        const double alt = tv == TINF ? 0.0 : (fin ? TINF : tv);
+        const double dz = d + ZOK;
+        const double y = ok ? d : 1.0;
+        t[b][k] = ok ? dz : alt;
+        P[b] = b == 0 ? y : A::mulmod(P[b - 1], y, M[k]);
       }
       double r = A::modinv(canon(P[B - 1], M[k]), M[k]);
 #pragma unroll
       for (int b = B - 1; b >= 0; --b) {
         const double z = t[b][k];
         const bool ok = z > 0.5 * ZOK;
+        const double zy = z - ZOK;
         const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
-        if (b > 0) r = A::mulmod(r, ok ? z - ZOK : 1.0, M[k]);
+        if (b > 0) r = A::mulmod(r, ok ? zy : 1.0, M[k]);
         const double tn = tsigned(A::mulmod(load_after(S.pp, b, inv), inv, M[k]), M[k]);
         const double tv2 = ok ? tn : z;
         t[b][k] = tv2;
-        const double kk = elim_key(tv2, ck[k]);
-        key[b] = k == 0 ? kk : __builtin_fmin(key[b], kk);
+        key[b] = elim_key(tv2, ck[k]);
       }
+      wkey = __builtin_fmin(wkey, wave_min_scatter<B>(key));
     }
-    const double kr = reduce_keys();
+    const double kr = reduce_keys(wkey);
     if (wave == 0) {
       if (lane < npairs && ps.live) {
         ps.remaining -= 1;                                          // :390
@@ -512,8 +532,12 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
           uint32_t idx;
           int32_t val;
           elim_decode(key_bits(kr), idx, val);
-          if (val == INT32_MIN) ps.live = 0;  // infinity: done (:402)
-          else publish_pivot(args, args.pairs[Gd.first + lane], lead, lane, ps, idx, val, S);
+          if (val == INT32_MIN) {
+            ps.live = 0;  // infinity: done (:402)
+          } else {
+            publish_pivot(args, args.pairs[Gd.first + lane], lead, lane, ps, idx, val, S);
+            mark_retire(lane, idx);
+          }
         }
       }
       if (lane < (uint32_t)B && !ps.live) {
@@ -822,17 +846,34 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   }
 }
 
+// Next work item for this workgroup (thread 0): queue 0, then the queue of
+// the XCD the workgroup runs on, then the others; exhausted queues are
+// remembered in *done.
+__device__ uint32_t next_item(const KernelArgs& args, uint32_t own, uint32_t* done) {
+  for (uint32_t s = 0; s < args.nqueues; ++s) {
+    const uint32_t q = s == 0 ? 0u : 1u + (own + s - 1) % (args.nqueues - 1);
+    if ((*done >> q) & 1) continue;
+    const uint32_t i = atomicAdd(args.queue + q * kQueueStride, 1u);
+    if (i < args.qoff[q + 1] - args.qoff[q]) return args.qoff[q] + i;
+    *done |= 1u << q;
+  }
+  return ~0u;
+}
+
 template <int T, int K, int B, class A>
 __global__ __launch_bounds__(T) void gcd_kernel(KernelArgs args) {
   __shared__ Shared<T, K, B> S;
   Ctx C;
   C.spin_limit = args.spin_limit;
+  const uint32_t xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);  // hwreg(HW_REG_XCC_ID, 0, 4)
+  const uint32_t own = args.nqueues > 1 ? xcc % (args.nqueues - 1) : 0;
+  uint32_t done = 0;
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) S.item = atomicAdd(args.queue, 1u);
+    if (threadIdx.x == 0) S.item = next_item(args, own, &done);
     __syncthreads();
     const uint32_t it = S.item;
-    if (it >= args.nitems) break;
+    if (it == ~0u) break;
     const WorkItem W = args.items[it];
     const GroupDesc Gd = args.groups[W.group];
     C.G = Gd.G;

@@ -189,6 +189,8 @@ struct mgcd_ctx {
 struct LaunchSet {
   int variant = -1;
   uint32_t nitems = 0;
+  uint32_t nqueues = 1;
+  uint32_t qoff[kMaxQueues + 1] = {0};
   uint32_t grid = 0;
   size_t nbar = 0;       // barrier slots
   size_t ndigits = 0;    // digit slots
@@ -245,9 +247,45 @@ int plan_items(mgcd_ctx* ctx, int vi, std::vector<PairDesc>& pd, std::vector<uin
     sorted.push_back(p);
   }
   pd.swap(sorted);
+  // Queues: groups of more than maxres/16 parts go to the shared queue 0;
+  // the others are spread over the 8 XCD queues, largest work first onto
+  // the least loaded.  At any time each queue has at most one group whose
+  // parts are not all taken; with the shared queue popped first this
+  // cannot tie up every workgroup as long as the shared queue's groups
+  // need at most maxres/2 workgroups.  Otherwise one queue holds everything.
+  const uint32_t nitems_all = [&] {
+    uint32_t n = 0;
+    for (const GroupDesc& gd : groups) n += gd.G;
+    return n;
+  }();
+  const uint32_t grid = std::min<uint32_t>(nitems_all, maxres);
+  const uint32_t small = grid / 16;
+  std::vector<std::vector<uint32_t>> qg(kMaxQueues);
+  uint32_t nqueues = maxG <= grid / 2 && grid >= 64 ? (uint32_t)kMaxQueues : 1u;
+  if (nqueues == 1) {
+    for (uint32_t gi = 0; gi < groups.size(); ++gi) qg[0].push_back(gi);
+  } else {
+    std::vector<double> load(kMaxQueues, 0.0);
+    for (uint32_t gi = 0; gi < groups.size(); ++gi) {  // groups are in decreasing N_Q order
+      const GroupDesc& gd = groups[gi];
+      if (gd.G > small) {
+        qg[0].push_back(gi);
+        continue;
+      }
+      const double w = (double)gd.count * gd.nq * gd.nq;
+      const int q = 1 + (int)(std::min_element(load.begin() + 1, load.end()) - load.begin() - 1);
+      load[q] += w;
+      qg[q].push_back(gi);
+    }
+  }
   items.clear();
-  for (uint32_t gi = 0; gi < groups.size(); ++gi)
-    for (uint32_t g = 0; g < groups[gi].G; ++g) items.push_back(WorkItem{gi, g});
+  for (uint32_t q = 0; q < nqueues; ++q) {
+    ls.qoff[q] = (uint32_t)items.size();
+    for (uint32_t gi : qg[q])
+      for (uint32_t g = 0; g < groups[gi].G; ++g) items.push_back(WorkItem{gi, g});
+  }
+  ls.qoff[nqueues] = (uint32_t)items.size();
+  ls.nqueues = nqueues;
   ls.variant = vi;
   ls.nitems = (uint32_t)items.size();
   ls.grid = std::min<uint32_t>(ls.nitems, maxres);
@@ -263,9 +301,11 @@ int plan_items(mgcd_ctx* ctx, int vi, std::vector<PairDesc>& pd, std::vector<uin
 // kernel cooperatively on ctx->stream; records events around the kernel.
 int launch(mgcd_ctx* ctx, const LaunchSet& ls, const KernelArgs& ka) {
   const KernelVariant& kv = kernel_variant(ls.variant);
-  HIP_TRY(launch_fill_u32(ka.queue, 1, 0u, ctx->stream));
+  HIP_TRY(launch_fill_u32(ka.queue, kMaxQueues * kQueueStride, 0u, ctx->stream));
   if (ls.nbar) HIP_TRY(launch_fill_u64(ka.bar, ls.nbar, 0ull, ctx->stream));
   KernelArgs args = ka;
+  args.nqueues = ls.nqueues;
+  std::memcpy(args.qoff, ls.qoff, sizeof args.qoff);
   void* params[] = {&args};
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
   HIP_TRY(hipLaunchCooperativeKernel(kv.fn, dim3(ls.grid), dim3(kv.T), params, 0, ctx->stream));
@@ -495,7 +535,7 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
 
   if (ctx->limbs.ensure((nu + nv + 1) * 4) != hipSuccess || ctx->pairs.ensure(sizeof(PairDesc)) != hipSuccess ||
       ctx->groups.ensure(sizeof(GroupDesc)) != hipSuccess ||
-      ctx->items.ensure(items.size() * sizeof(WorkItem)) != hipSuccess || ctx->queue.ensure(4) != hipSuccess ||
+      ctx->items.ensure(items.size() * sizeof(WorkItem)) != hipSuccess || ctx->queue.ensure(kMaxQueues * kQueueStride * 4) != hipSuccess ||
       ctx->bar.ensure((ls.nbar + 1) * 8) != hipSuccess || ctx->digits.ensure(ls.ndigits * 8) != hipSuccess ||
       ctx->trace.ensure((ls.ntrace + 1) * 8) != hipSuccess || ctx->res.ensure((ls.nres + 1) * 4) != hipSuccess ||
       ctx->out.ensure(sizeof(PairOut)) != hipSuccess)
@@ -512,7 +552,6 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
   ka.pairs = ctx->pairs.as<PairDesc>();
   ka.groups = ctx->groups.as<GroupDesc>();
   ka.items = ctx->items.as<WorkItem>();
-  ka.nitems = ls.nitems;
   ka.L = ctx->L;
   ka.queue = ctx->queue.as<uint32_t>();
   ka.mod_u32 = ctx->d_mod_u32.as<uint32_t>();
@@ -674,7 +713,7 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
   int T = opts_workgroup, K = opts_slots, Bp = opts_group;
   if (T <= 0 || K <= 0 || Bp <= 0) {
     T = 256;
-    K = 1;
+    K = 2;
     Bp = 16;
   }
   const int vi = find_kernel_variant(T, K, Bp, ctx->wide);
@@ -691,7 +730,7 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
     const size_t np = b->pd.size();
     if (b->limbs.ensure(packed.size() * 4 + 4) != hipSuccess || b->pairs.ensure(np * sizeof(PairDesc)) != hipSuccess ||
         b->groups_d.ensure(b->groups.size() * sizeof(GroupDesc)) != hipSuccess ||
-        b->items_d.ensure(b->items.size() * sizeof(WorkItem)) != hipSuccess || b->queue.ensure(4) != hipSuccess ||
+        b->items_d.ensure(b->items.size() * sizeof(WorkItem)) != hipSuccess || b->queue.ensure(kMaxQueues * kQueueStride * 4) != hipSuccess ||
         b->bar.ensure((b->ls.nbar + 1) * 8) != hipSuccess || b->digits.ensure(b->ls.ndigits * 8) != hipSuccess ||
         b->trace.ensure(b->ls.ntrace * 8) != hipSuccess || b->res.ensure(b->ls.nres * 4) != hipSuccess ||
         b->out.ensure(np * sizeof(PairOut)) != hipSuccess) {
@@ -726,7 +765,6 @@ int mgcd_batch_run(mgcd_batch* b) {
     ka.pairs = b->pairs.as<PairDesc>();
     ka.groups = b->groups_d.as<GroupDesc>();
     ka.items = b->items_d.as<WorkItem>();
-    ka.nitems = b->ls.nitems;
     ka.L = ctx->L;
     ka.queue = b->queue.as<uint32_t>();
     ka.mod_u32 = ctx->d_mod_u32.as<uint32_t>();
