@@ -64,14 +64,22 @@ def main():
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["headline", "mixed", "skewed", "serve"], default="headline",
+                    help="headline: the metric's 100k-bit batch (default); mixed: 4096 pairs of 1k-200k bits, "
+                         "autotuned; skewed: 16384 pairs, 95%% 4k-bit and 5%% 500k-bit, 27-bit moduli, replayed "
+                         "hipGraph; serve: Poisson arrivals into the serving loop at five offered rates")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the modular GCD path is not sharded across GPUs "
-              "(see DESIGN.md, 'Sharding').", file=sys.stderr)
+        print("bench.py runs on one GPU only: the modular GCD path is not sharded across GPUs; "
+              "a batch of pairs would be split across GPUs pair by pair, which needs no exchange.",
+              file=sys.stderr)
         sys.exit(2)
 
     m = ge.package()
     w = ge.workload()
+    if args.workload != "headline":
+        {"mixed": run_mixed, "skewed": run_skewed, "serve": run_serve}[args.workload](args, m, w)
+        return
     t0 = time.time()
     pairs = w.headline(args.pairs, seed=7, nbits=args.bits, gbits=args.gbits)
     log(f"generated {len(pairs)} pairs in {time.time() - t0:.1f}s")
@@ -183,6 +191,164 @@ def main():
     print(json.dumps(line), flush=True)
     batch.close()
     ctx.close()
+
+
+def _one_per_launch(ctx, pairs, n, seed):
+    """Mean seconds per pair of the one-pair path on a seeded sample."""
+    import random
+    rnd = random.Random(seed)
+    sample = rnd.sample(range(len(pairs)), min(n, len(pairs)))
+    lat = []
+    for i in sample:
+        u, v = pairs[i]
+        t = time.perf_counter()
+        g = ctx.gcd(u, v)
+        lat.append(time.perf_counter() - t)
+        assert g == math.gcd(u, v), f"pair {i}: one-per-launch result differs from math.gcd"
+    lat.sort()
+    return statistics.mean(lat), statistics.median(lat), lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)]
+
+
+def _timed_runs(batch, ctx, steps, warmup):
+    for _ in range(warmup):
+        batch.run()
+    ctx.synchronize()
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+        dev.append(batch.timing()[0])
+    ctx.synchronize()
+    return (time.perf_counter() - t0) / steps, statistics.mean(dev)
+
+
+def run_mixed(args, m, w):
+    """4096 pairs, log-uniform 1k..200k bits, g uniform in [0, half], 10% coprime, 1% v = 0, L = 32;
+    size classes autotuned on first use; against one pair per launch."""
+    pairs = w.mixed(4096, seed=3)
+    ctx = m.Context(32)
+    ctx.set_option(m.OPT_AUTOTUNE, 1)
+    t0 = time.perf_counter()
+    batch = m.Batch.from_pairs(ctx, pairs)
+    tune_s = time.perf_counter() - t0
+    step_s, kern_s = _timed_runs(batch, ctx, args.steps, args.warmup)
+    res = batch.results()
+    for i, (u, v) in enumerate(pairs):
+        assert res[i] == math.gcd(u, v), f"pair {i}: result differs from math.gcd"
+    launches = batch.launches()
+    mean1, p50, p99 = _one_per_launch(ctx, pairs, args.per_launch_sample * 4, 11)
+    value = len(pairs) / step_s
+    print(json.dumps({
+        "metric": "GCDs/sec batched vs one-per-launch, 4096 mixed-size pairs; bit-exact",
+        "value": round(value, 3), "unit": "GCD/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 3), "kernel_ms": round(kern_s * 1e3, 3),
+        "config": {"workload": "4096 pairs, bit lengths log-uniform in [1k, 200k], g bits uniform in [0, len/2], "
+                               "10% coprime, 1% v = 0, 32-bit moduli", "seed": 3},
+        "launches": [{"tiling": x["tiling"], "pairs": x["pairs"], "nq": x["nq"],
+                      "ms": round(x["device_s"] * 1e3, 3)} for x in launches],
+        "autotune_s": round(tune_s, 2),
+        "one_per_launch": {"gcds_per_s": round(1 / mean1, 3), "p50_ms": round(p50 * 1e3, 3),
+                           "p99_ms": round(p99 * 1e3, 3), "sample": args.per_launch_sample * 4},
+        "batched_vs_one_per_launch": round(value * mean1, 2),
+        "bit_exact": {"math_gcd_equal": len(pairs)},
+    }), flush=True)
+    batch.close()
+    ctx.close()
+
+
+def run_skewed(args, m, w):
+    """16384 pairs: 95% 4k-bit, 5% 500k-bit, g = 20% of the length, L = 27; the batch's launch
+    sequence is captured once into a hipGraph and replayed."""
+    pairs = w.skewed(16384, seed=4)
+    ctx = m.Context(27)
+    eager = m.Batch.from_pairs(ctx, pairs)
+    eager.run()
+    eager.run()
+    launches = eager.launches()
+    tot = sum(x["device_s"] for x in launches)
+    res = eager.results()
+    eager.close()
+    ctx.set_option(m.OPT_BATCH_GRAPH, 1)
+    batch = m.Batch.from_pairs(ctx, pairs)
+    ctx.set_option(m.OPT_BATCH_GRAPH, 0)
+    step_s, kern_s = _timed_runs(batch, ctx, args.steps, args.warmup)
+    assert batch.results() == res, "graph replay differs from the eager run"
+    for i, (u, v) in enumerate(pairs):
+        assert res[i] == math.gcd(u, v), f"pair {i}: result differs from math.gcd"
+    value = len(pairs) / step_s
+    print(json.dumps({
+        "metric": "GCDs/sec, skewed batch (95% 4k-bit, 5% 500k-bit) replayed as a hipGraph; bit-exact",
+        "value": round(value, 3), "unit": "GCD/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 3), "kernel_ms": round(kern_s * 1e3, 3),
+        "config": {"workload": "16384 pairs, 95% 4096-bit and 5% 500000-bit, g = 20% of the length, "
+                               "27-bit moduli", "seed": 4},
+        "launches_eager": [{"tiling": x["tiling"], "pairs": x["pairs"], "nq": x["nq"],
+                            "ms": round(x["device_s"] * 1e3, 3),
+                            "share": round(x["device_s"] / tot, 4) if tot else None} for x in launches],
+        "bit_exact": {"math_gcd_equal": len(pairs)},
+    }), flush=True)
+    batch.close()
+    ctx.close()
+
+
+def run_serve(args, m, w):
+    """Poisson arrivals at five offered rates; sizes 70% 2k, 25% 50k, 5% 1M bits, g = 10% of the
+    length; 64 GB residue budget; two launch slots."""
+    import random
+    import threading
+    rates = [25, 50, 100, 200, 400]
+    n = 300
+    out = []
+    ctx = m.Context(32)
+    for ri, rate in enumerate(rates):
+        sizes = w.serve_sizes(n, seed=5 + ri)
+        reqs = [w.serve_pair(100 + ri, i, nb) for i, nb in enumerate(sizes)]
+        rnd = random.Random(ri)
+        gaps = [rnd.expovariate(rate) for _ in range(n)]
+        srv = m.Server(ctx, budget_bytes=64 << 30, slots=2)
+        tickets = [None] * n
+
+        def feed():
+            t = time.perf_counter()
+            for i in range(n):
+                t += gaps[i]
+                d = t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = srv.submit(*reqs[i])
+
+        th = threading.Thread(target=feed)
+        t0 = time.perf_counter()
+        th.start()
+        th.join()
+        lat = []
+        for i in range(n):
+            g, l = srv.wait(tickets[i])
+            assert g == math.gcd(*reqs[i]), f"rate {rate} request {i}: result differs from math.gcd"
+            lat.append(l)
+        span = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+
+        def pct(xs, f):
+            xs = sorted(xs)
+            return round(xs[min(len(xs) - 1, int(math.ceil(f * len(xs))) - 1)] * 1e3, 3)
+
+        per_size = {str(nb): {"n": sizes.count(nb), "p50_ms": pct([l for l, s2 in zip(lat, sizes) if s2 == nb], 0.5),
+                              "p99_ms": pct([l for l, s2 in zip(lat, sizes) if s2 == nb], 0.99)}
+                    for nb in sorted(set(sizes))}
+        out.append({"offered_per_s": rate, "completed": n, "achieved_per_s": round(n / span, 2),
+                    "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99), "by_bits": per_size,
+                    "peak_resident_mb": round(st["peak_bytes"] / 2**20, 1)})
+        log(f"rate {rate}/s: {out[-1]}")
+    ctx.close()
+    print(json.dumps({
+        "metric": "serving: p50/p99 latency and GCD/s at offered rates; bit-exact",
+        "unit": "ms", "n_gpus": 1, "higher_is_better": False,
+        "config": {"workload": f"Poisson arrivals, {n} requests per rate, sizes 70% 2k / 25% 50k / 5% 1M bits, "
+                               "g = 10% of the length, 32-bit moduli, 64 GB budget, 2 launch slots"},
+        "rates": out,
+    }), flush=True)
 
 
 if __name__ == "__main__":

@@ -90,7 +90,17 @@ enum {
   /* Tiling of the one-pair path: threads per workgroup and moduli slots per
    * thread (both 0, the default: chosen from N_Q). */
   MGCD_OPT_SINGLE_WORKGROUP = 3,
-  MGCD_OPT_SINGLE_SLOTS = 4
+  MGCD_OPT_SINGLE_SLOTS = 4,
+  /* 1: batches created with tiling (0, 0, 0) measure the candidate tilings
+   * on a sample of each size class (mgcd_autotune) unless a tuned tiling for
+   * that class is cached in the context or in the file named by
+   * $MGCD_TUNE_CACHE, which receives the new results.  0 (default): cached
+   * or built-in tilings. */
+  MGCD_OPT_AUTOTUNE = 5,
+  /* 1: mgcd_batch_run captures the batch's launch sequence into a hipGraph
+   * on its first run and replays the graph afterwards.  0 (default):
+   * cooperative launches. */
+  MGCD_OPT_BATCH_GRAPH = 6
 };
 int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value);
 /* Blocks until all work queued on the context's device has finished. */
@@ -135,10 +145,53 @@ int mgcd_batch_timing(const mgcd_batch* b, double* device_s, double* host_s);
 /* Work done for pair i in the last run: N_Q, elimination iterations R and
  * recovery digits k (0 for pairs answered on the host). */
 int mgcd_batch_pair_stats(const mgcd_batch* b, size_t i, uint32_t* nq, uint32_t* iters, uint32_t* digits);
-/* The tiling chosen: threads per workgroup, moduli slots per thread, pairs
- * per group. */
+/* The tiling of the launch with the most pairs: threads per workgroup,
+ * moduli slots per thread, pairs per group. */
 int mgcd_batch_tiling(const mgcd_batch* b, int* workgroup, int* slots, int* group);
+/* A batch created with tiling (0, 0, 0) is split into size classes of N_Q
+ * (up to 1024, 4096, 16384, more), one kernel launch each with its own
+ * tiling; an explicit tiling gives one launch.  Launch i: tiling, pairs,
+ * N_Q range and the seconds its kernel took in the last run (0 when the run
+ * replayed a graph). */
+int mgcd_batch_launch_count(const mgcd_batch* b, size_t* count);
+int mgcd_batch_launch_info(const mgcd_batch* b, size_t i, int* workgroup, int* slots, int* group, size_t* npairs,
+                           uint32_t* nq_min, uint32_t* nq_max, double* device_s);
 int mgcd_batch_destroy(mgcd_batch* b);
+
+/* Autotuner: runs the batch kernel with each candidate tiling on these
+ * pairs (one warm-up and one timed run each) and returns the fastest.
+ * No counterpart in the reference, which has one fixed launch shape
+ * (GmpCudaDevice-gcd.cu:80-84). */
+int mgcd_autotune(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, const uint32_t* nu,
+                  const uint64_t* v_off, const uint32_t* nv, size_t npairs, int* workgroup, int* slots, int* group);
+
+/* Serving loop.  Requests of any size are submitted one by one; a worker
+ * thread admits waiting requests in arrival order while their device
+ * footprint (limbs, residues, pivots, digits: about 24 N_Q + 4 (nu + nv)
+ * bytes each) fits budget_bytes, packs them into the next launch of the
+ * persistent kernel, and keeps up to `slots` launches in flight on their
+ * own streams (each with a 1/slots share of the device's workgroups), so
+ * requests that arrive while others run are started as soon as a slot and
+ * memory free up and overlap with them on the device.  The context belongs
+ * to the server's worker until mgcd_server_destroy.  No counterpart in the
+ * reference (one blocking call per pair, GmpCudaDevice-gcd.cu:57-132).
+ *   submit: copies the operands; MGCD_ERR_BUSY if the request alone
+ *           exceeds the budget.
+ *   wait:   blocks until ticket's gcd is known (timeout_s < 0: no limit;
+ *           MGCD_ERR_TIMEOUT after timeout_s seconds otherwise), then
+ *           returns its status, its limbs in g and the latency from submit
+ *           to completion, and releases the ticket (kept on
+ *           MGCD_ERR_CAPACITY).
+ *   stats:  completed requests, p50 and p99 latency, completions per
+ *           second since the first submit, peak resident bytes. */
+int mgcd_server_create(mgcd_ctx* ctx, uint64_t budget_bytes, int slots, mgcd_server** out);
+int mgcd_server_submit(mgcd_server* s, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv,
+                       uint64_t* ticket);
+int mgcd_server_wait(mgcd_server* s, uint64_t ticket, double timeout_s, uint32_t* g, size_t cap, size_t* ng,
+                     double* latency_s);
+int mgcd_server_stats(mgcd_server* s, uint64_t* completed, double* p50_s, double* p99_s, double* gcds_per_s,
+                      uint64_t* peak_bytes);
+int mgcd_server_destroy(mgcd_server* s);
 
 /* Exhaustive certification of the FP32-reciprocal quasi-quotient on this
  * device (the role of certifyQuoRemQuasi.cu):

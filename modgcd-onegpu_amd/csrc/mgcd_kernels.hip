@@ -266,23 +266,43 @@ __device__ double grid_min(double x, Ctx& C) {
   return __ballot(late) ? KNONE : m;
 }
 
-// (a) Residues of x (n limbs) for every slot row in use.
-template <int T, int K>
-__device__ void residues(const uint32_t* __restrict__ x, uint32_t n, double (&r)[K], const ModD (&M)[K],
-                         int rows, double* tile) {
+// (a) Residues of NOP operands at once (x[o], n[o] limbs) for every slot
+// row in use: NOP * K independent Horner chains per lane keep the FP64
+// pipeline busy.  The operands are aligned at their least significant limb
+// and padded with leading zero limbs, which leave a Horner residue
+// unchanged, so every chain takes the same steps.  Limbs are staged
+// through LDS, TILE / NOP per operand at a time, read from HBM coalesced.
+template <int T, int K, int NOP>
+__device__ void residues(const uint32_t* (&x)[NOP], const uint32_t (&n)[NOP], double (&r)[NOP][K],
+                         const ModD (&M)[K], int rows, double* tile) {
+  constexpr int TS = TILE / NOP;
+  uint32_t nmax = 0;
 #pragma unroll
-  for (int k = 0; k < K; ++k) r[k] = 0.0;
-  for (int hi = (int)n; hi > 0; hi -= TILE) {
-    const int lo = hi > TILE ? hi - TILE : 0;
-    const int len = hi - lo;
-    __syncthreads();
-    for (int j = threadIdx.x; j < len; j += T) tile[j] = (double)x[lo + j];
-    __syncthreads();
-    for (int j = len - 1; j >= 0; --j) {
-      const double limb = tile[j];
+  for (int o = 0; o < NOP; ++o) {
+    nmax = max(nmax, n[o]);
 #pragma unroll
-      for (int k = 0; k < K; ++k)
-        if (k < rows) r[k] = horner_step(r[k], limb, M[k]);
+    for (int k = 0; k < K; ++k) r[o][k] = 0.0;
+  }
+  // Step s handles limb index nmax - 1 - s of every operand.
+  for (uint32_t s0 = 0; s0 < nmax; s0 += TS) {
+    const uint32_t len = min((uint32_t)TS, nmax - s0);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < len * NOP; j += T) {
+      const uint32_t o = j / len, jj = j - o * len;
+      const uint32_t i = nmax - 1 - (s0 + jj);
+#pragma unroll
+      for (int q = 0; q < NOP; ++q)
+        if ((uint32_t)q == o) tile[q * TS + jj] = i < n[q] ? (double)x[q][i] : 0.0;
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < len; ++j) {
+#pragma unroll
+      for (int o = 0; o < NOP; ++o) {
+        const double limb = tile[o * TS + j];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (k < rows) r[o][k] = horner_step(r[o][k], limb, M[k]);
+      }
     }
   }
 }
@@ -345,17 +365,32 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   }
 
   // ---- (a) residues of u and v, stored canonical for (c) -----------------
-  for (uint32_t b = 0; b < npairs; ++b) {
-    const PairDesc& P = args.pairs[Gd.first + b];
-    double r[K];
-    residues<T, K>(args.limbs + P.u_off, P.nu, r, M, rows, S.tile);
+  // PP pairs (2 PP operands) at a time.
+  constexpr int PP = (B >= 2 && K <= 2) ? 2 : 1;
+  for (uint32_t b0 = 0; b0 < npairs; b0 += PP) {
+    const uint32_t* x[2 * PP];
+    uint32_t nx[2 * PP];
+    uint64_t ro[2 * PP];
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-      if ((uint32_t)k * T + tid < cnt) args.res[P.res_off + base + k * T + tid] = (uint32_t)canon(r[k], M[k]);
-    residues<T, K>(args.limbs + P.v_off, P.nv, r, M, rows, S.tile);
+    for (int q = 0; q < PP; ++q) {
+      const bool here = b0 + q < npairs;
+      const PairDesc& P = args.pairs[Gd.first + (here ? b0 + q : b0)];
+      x[2 * q] = args.limbs + P.u_off;
+      x[2 * q + 1] = args.limbs + P.v_off;
+      nx[2 * q] = here ? P.nu : 0u;
+      nx[2 * q + 1] = here ? P.nv : 0u;
+      ro[2 * q] = P.res_off;
+      ro[2 * q + 1] = P.res_off + nq;
+    }
+    double r[2 * PP][K];
+    residues<T, K, 2 * PP>(x, nx, r, M, rows, S.tile);
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-      if ((uint32_t)k * T + tid < cnt) args.res[P.res_off + nq + base + k * T + tid] = (uint32_t)canon(r[k], M[k]);
+    for (int o = 0; o < 2 * PP; ++o) {
+      if (b0 + o / 2 >= npairs) break;
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if ((uint32_t)k * T + tid < cnt) args.res[ro[o] + base + k * T + tid] = (uint32_t)canon(r[o][k], M[k]);
+    }
   }
 
   // ---- (b) elimination ----------------------------------------------------
@@ -957,7 +992,8 @@ __global__ void certify_endpoints_kernel(int L, uint64_t y_lo, uint64_t y_hi, in
   MGCD_VARIANT(256, 1, 1, A, W), MGCD_VARIANT(256, 2, 1, A, W), MGCD_VARIANT(256, 4, 1, A, W),           \
       MGCD_VARIANT(256, 8, 1, A, W), MGCD_VARIANT(1024, 1, 1, A, W), MGCD_VARIANT(1024, 2, 1, A, W),     \
       MGCD_VARIANT(256, 1, 8, A, W), MGCD_VARIANT(256, 2, 8, A, W), MGCD_VARIANT(256, 1, 16, A, W),      \
-      MGCD_VARIANT(256, 2, 16, A, W), MGCD_VARIANT(256, 1, 32, A, W), MGCD_VARIANT(128, 1, 16, A, W)
+      MGCD_VARIANT(256, 2, 16, A, W), MGCD_VARIANT(256, 1, 32, A, W), MGCD_VARIANT(128, 1, 16, A, W),    \
+      MGCD_VARIANT(512, 2, 16, A, W), MGCD_VARIANT(512, 1, 16, A, W)
 static const KernelVariant kVariants[] = {MGCD_VARIANTS(Arith32, 1), MGCD_VARIANTS(Arith22, 0)};
 #undef MGCD_VARIANTS
 #undef MGCD_VARIANT

@@ -12,12 +12,16 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "mgcd_kernels.h"
@@ -183,6 +187,10 @@ struct mgcd_ctx {
   bool retry = true;      // MGCD_OPT_RETRY
   int single_T = 0;       // MGCD_OPT_SINGLE_WORKGROUP (0: automatic)
   int single_K = 0;       // MGCD_OPT_SINGLE_SLOTS (0: automatic)
+  bool autotune = false;  // MGCD_OPT_AUTOTUNE
+  bool batch_graph = false;  // MGCD_OPT_BATCH_GRAPH
+  bool tune_loaded = false;
+  std::map<std::string, int> tuned;  // size class -> kernel variant
 };
 
 // A prepared launch over device-resident pairs.
@@ -297,6 +305,23 @@ int plan_items(mgcd_ctx* ctx, int vi, std::vector<PairDesc>& pd, std::vector<uin
   return MGCD_OK;
 }
 
+// Launches a persistent kernel whose grid fits the device's resident
+// capacity for it.  Cooperatively when asked and possible: that guarantees
+// that all workgroups are resident.  Where the runtime refuses a
+// cooperative launch (another process holds the device's cooperative
+// queue, or a graph is being captured) the kernel is launched plainly with
+// the same grid; on an otherwise idle device its workgroups are then all
+// resident as well, and a workgroup that is not only delays the others'
+// bounded waits, which end in a timeout status that the host retries.
+hipError_t launch_resident(const void* fn, uint32_t grid, int T, void** params, hipStream_t s, bool cooperative) {
+  if (cooperative) {
+    const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(T), params, 0, s);
+    if (e == hipSuccess) return e;
+    (void)hipGetLastError();
+  }
+  return hipLaunchKernel(fn, dim3(grid), dim3(T), params, 0, s);
+}
+
 // Clears the queue counter and barrier slots, then launches the persistent
 // kernel cooperatively on ctx->stream; records events around the kernel.
 int launch(mgcd_ctx* ctx, const LaunchSet& ls, const KernelArgs& ka) {
@@ -308,7 +333,7 @@ int launch(mgcd_ctx* ctx, const LaunchSet& ls, const KernelArgs& ka) {
   std::memcpy(args.qoff, ls.qoff, sizeof args.qoff);
   void* params[] = {&args};
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-  HIP_TRY(hipLaunchCooperativeKernel(kv.fn, dim3(ls.grid), dim3(kv.T), params, 0, ctx->stream));
+  HIP_TRY(launch_resident(kv.fn, ls.grid, kv.T, params, ctx->stream, true));
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
   return MGCD_OK;
 }
@@ -489,6 +514,12 @@ int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value) {
     case MGCD_OPT_SINGLE_SLOTS:
       ctx->single_K = (int)value;
       return MGCD_OK;
+    case MGCD_OPT_AUTOTUNE:
+      ctx->autotune = value != 0;
+      return MGCD_OK;
+    case MGCD_OPT_BATCH_GRAPH:
+      ctx->batch_graph = value != 0;
+      return MGCD_OK;
     default:
       return MGCD_ERR_ARG;
   }
@@ -643,24 +674,204 @@ int mgcd_ctx_synchronize(mgcd_ctx* ctx) {
 
 // ===========================================================================
 // Batches.
+//
+// A batch is split into size classes by N_Q; each class is one launch of
+// the persistent kernel with its own tiling (explicit, tuned, or the
+// default), all on the context's stream.  With MGCD_OPT_BATCH_GRAPH the
+// launch sequence (queue and barrier fills, kernels) is captured into a
+// hipGraph on the first run and replayed afterwards.
+
+namespace {
+
+// Upper N_Q bound of each size class (the last class is unbounded).
+constexpr uint32_t kClassMaxNq[] = {1024, 4096, 16384, 0xFFFFFFFFu};
+constexpr int kNumClasses = 4;
+
+int class_of(uint32_t nq) {
+  int c = 0;
+  while (nq > kClassMaxNq[c]) ++c;
+  return c;
+}
+
+// Batch tilings the tuner tries (threads, slots, pairs per group).
+constexpr int kTuneCandidates[][3] = {{256, 2, 16}, {512, 2, 16}, {256, 1, 16}, {512, 1, 16},
+                                     {256, 2, 8},  {256, 1, 8},  {256, 1, 32}, {128, 1, 16}};
+
+int default_variant(const mgcd_ctx* ctx, int cls) {
+  return cls == 0 ? find_kernel_variant(256, 1, 16, ctx->wide) : find_kernel_variant(256, 2, 16, ctx->wide);
+}
+
+}  // namespace
+
+struct BatchLaunch {
+  int variant = -1;
+  LaunchSet ls;
+  size_t pair_first = 0, pair_count = 0;
+  size_t item_first = 0;
+  size_t queue_first = 0;  // counters of this launch's queues
+  size_t bar_first = 0, nbar = 0;
+  uint32_t nq_min = 0, nq_max = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double device_s = 0.0;
+};
+
 struct mgcd_batch {
   mgcd_ctx* ctx = nullptr;
   size_t npairs = 0;
-  std::vector<PairDesc> pd;          // kernel pairs
+  std::vector<PairDesc> pd;          // kernel pairs, launch by launch, in group order
   std::vector<int64_t> kidx;         // pair i -> kernel pair index, or -1 (handled on host)
   std::vector<std::vector<uint32_t>> host_result;  // results of pairs handled on host / converted
   std::vector<int> status;
-  std::vector<uint32_t> nv_limbs;    // capacity hint per pair
-  std::vector<WorkItem> items;
-  LaunchSet ls;
+  std::vector<BatchLaunch> launches;
   std::vector<GroupDesc> groups;
+  std::vector<WorkItem> items;
+  size_t ndigits = 0, ntrace = 0, nres = 0, nbar = 0;
   DevBuf limbs, pairs, groups_d, items_d, queue, bar, digits, trace, res, out;
   std::vector<PairOut> h_out;
   std::vector<mgcd_pair_t> h_digits;
   double device_s = 0.0, host_s = 0.0;
   bool ran = false;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   std::vector<uint32_t> packed;      // host copy of the kernel pairs' limbs (for retries)
 };
+
+namespace {
+
+KernelArgs batch_args(const mgcd_batch* b, const BatchLaunch& l) {
+  const mgcd_ctx* ctx = b->ctx;
+  KernelArgs ka;
+  std::memset(&ka, 0, sizeof ka);
+  ka.limbs = b->limbs.as<uint32_t>();
+  ka.pairs = b->pairs.as<PairDesc>();
+  ka.groups = b->groups_d.as<GroupDesc>();
+  ka.items = b->items_d.as<WorkItem>() + l.item_first;
+  ka.L = ctx->L;
+  ka.queue = b->queue.as<uint32_t>() + l.queue_first;
+  ka.mod_u32 = ctx->d_mod_u32.as<uint32_t>();
+  ka.mod_d = ctx->d_mod_d.as<double>();
+  ka.mod_inv = ctx->d_mod_inv.as<double>();
+  ka.bar = b->bar.as<unsigned long long>();
+  ka.digits = b->digits.as<mgcd_pair_t>();
+  ka.trace = b->trace.as<mgcd_pair_t>();
+  ka.res = b->res.as<uint32_t>();
+  ka.out = b->out.as<PairOut>();
+  ka.spin_limit = kSpinLimit;
+  ka.nqueues = l.ls.nqueues;
+  std::memcpy(ka.qoff, l.ls.qoff, sizeof ka.qoff);
+  return ka;
+}
+
+// Queues the fills and the kernel of one launch on ctx->stream.  A
+// cooperative launch guarantees that every workgroup is resident; inside a
+// graph capture the kernel is launched plainly with the same grid, which
+// never exceeds the device's resident capacity for the variant.
+int enqueue_launch(mgcd_batch* b, const BatchLaunch& l, bool cooperative) {
+  mgcd_ctx* ctx = b->ctx;
+  KernelArgs ka = batch_args(b, l);
+  HIP_TRY(launch_fill_u32(ka.queue, kMaxQueues * kQueueStride, 0u, ctx->stream));
+  if (l.nbar) HIP_TRY(launch_fill_u64(ka.bar + l.bar_first, l.nbar, 0ull, ctx->stream));
+  const KernelVariant& kv = kernel_variant(l.variant);
+  void* params[] = {&ka};
+  HIP_TRY(launch_resident(kv.fn, l.ls.grid, kv.T, params, ctx->stream, cooperative));
+  return MGCD_OK;
+}
+
+// Plans the batch's launches: one per size class, or a single one for an
+// explicit variant.  b->pd is reordered launch by launch.
+int plan_batch(mgcd_batch* b, int forced_variant, const std::vector<int>& class_variant) {
+  mgcd_ctx* ctx = b->ctx;
+  std::vector<std::vector<uint32_t>> members(forced_variant >= 0 ? 1 : kNumClasses);
+  for (uint32_t j = 0; j < b->pd.size(); ++j) members[forced_variant >= 0 ? 0 : class_of(b->pd[j].nq)].push_back(j);
+  std::vector<PairDesc> all;
+  std::vector<int64_t> pos(b->pd.size(), -1);
+  size_t dig = 0, tr = 0, rs = 0, bar = 0;
+  for (size_t c = 0; c < members.size(); ++c) {
+    if (members[c].empty()) continue;
+    BatchLaunch l;
+    l.variant = forced_variant >= 0 ? forced_variant : class_variant[c];
+    if (l.variant < 0) return MGCD_ERR_ARG;
+    std::vector<PairDesc> cpd;
+    for (uint32_t j : members[c]) cpd.push_back(b->pd[j]);
+    std::vector<uint32_t> order;
+    std::vector<GroupDesc> cg;
+    std::vector<WorkItem> ci;
+    const int rc = plan_items(ctx, l.variant, cpd, order, cg, ci, l.ls);
+    if (rc != MGCD_OK) return rc;
+    l.pair_first = all.size();
+    l.pair_count = cpd.size();
+    l.item_first = b->items.size();
+    l.queue_first = b->launches.size() * (size_t)kMaxQueues * kQueueStride;
+    l.bar_first = bar;
+    l.nbar = l.ls.nbar;
+    l.nq_min = 0xFFFFFFFFu;
+    for (PairDesc& p : cpd) {
+      p.dig_off += dig;
+      p.trace_off += tr;
+      p.res_off += rs;
+      l.nq_min = std::min(l.nq_min, p.nq);
+      l.nq_max = std::max(l.nq_max, p.nq);
+    }
+    const size_t group_first = b->groups.size();
+    for (GroupDesc& g : cg) {
+      g.first += (uint32_t)l.pair_first;
+      g.bar_off += bar;
+      b->groups.push_back(g);
+    }
+    for (WorkItem& w : ci) {
+      w.group += (uint32_t)group_first;
+      b->items.push_back(w);
+    }
+    for (size_t j = 0; j < order.size(); ++j) pos[members[c][order[j]]] = (int64_t)(l.pair_first + j);
+    all.insert(all.end(), cpd.begin(), cpd.end());
+    dig += l.ls.ndigits;
+    tr += l.ls.ntrace;
+    rs += l.ls.nres;
+    bar += l.ls.nbar;
+    b->launches.push_back(l);
+  }
+  b->pd.swap(all);
+  for (auto& k : b->kidx)
+    if (k >= 0) k = pos[k];
+  b->ndigits = dig;
+  b->ntrace = tr;
+  b->nres = rs;
+  b->nbar = bar;
+  return MGCD_OK;
+}
+
+// Key of a tuned tiling: width L and size class.
+std::string tune_key(int L, int cls) { return std::to_string(L) + ":" + std::to_string(cls); }
+
+void load_tune_cache(mgcd_ctx* ctx) {
+  const char* path = std::getenv("MGCD_TUNE_CACHE");
+  if (!path || !*path || ctx->tune_loaded) return;
+  ctx->tune_loaded = true;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return;
+  char key[64];
+  int T, K, B;
+  while (std::fscanf(f, "%63s %d %d %d", key, &T, &K, &B) == 4) {
+    const int vi = find_kernel_variant(T, K, B, ctx->wide);
+    if (vi >= 0) ctx->tuned[key] = vi;
+  }
+  std::fclose(f);
+}
+
+void save_tune_cache(const mgcd_ctx* ctx) {
+  const char* path = std::getenv("MGCD_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return;
+  for (const auto& kv : ctx->tuned) {
+    const KernelVariant& v = kernel_variant(kv.second);
+    std::fprintf(f, "%s %d %d %d\n", kv.first.c_str(), v.T, v.K, v.B);
+  }
+  std::fclose(f);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -673,12 +884,12 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
   mgcd_batch* b = new mgcd_batch();
   b->ctx = ctx;
   b->npairs = npairs;
+  b->use_graph = ctx->batch_graph;
   b->kidx.assign(npairs, -1);
   b->host_result.resize(npairs);
   b->status.assign(npairs, MGCD_OK);
   // Pack the kernel pairs' limbs, larger operand first.
   std::vector<uint32_t>& packed = b->packed;
-  uint32_t max_nq = 0;
   for (size_t i = 0; i < npairs; ++i) {
     const uint32_t* a = limbs + u_off[i];
     const uint32_t* c = limbs + v_off[i];
@@ -705,34 +916,62 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
     p.nu = (uint32_t)na;
     p.nv = (uint32_t)nc;
     p.nq = (uint32_t)nqi;
-    max_nq = std::max(max_nq, p.nq);
     b->kidx[i] = (int64_t)b->pd.size();
     b->pd.push_back(p);
   }
-  // Tiling: explicit, or a default that fits the median pair in one workgroup.
-  int T = opts_workgroup, K = opts_slots, Bp = opts_group;
-  if (T <= 0 || K <= 0 || Bp <= 0) {
-    T = 256;
-    K = 2;
-    Bp = 16;
+  int rc = MGCD_OK;
+  int forced = -1;
+  std::vector<int> class_variant(kNumClasses, -1);
+  if (opts_workgroup > 0 && opts_slots > 0 && opts_group > 0) {
+    forced = find_kernel_variant(opts_workgroup, opts_slots, opts_group, ctx->wide);
+    if (forced < 0) rc = MGCD_ERR_ARG;
+  } else if (opts_workgroup || opts_slots || opts_group) {
+    rc = MGCD_ERR_ARG;
+  } else {
+    load_tune_cache(ctx);
+    std::vector<std::vector<size_t>> byclass(kNumClasses);
+    for (size_t i = 0; i < npairs; ++i)
+      if (b->kidx[i] >= 0) byclass[class_of(b->pd[b->kidx[i]].nq)].push_back(i);
+    bool tuned_any = false;
+    for (int c = 0; c < kNumClasses && rc == MGCD_OK; ++c) {
+      if (byclass[c].empty()) continue;
+      const auto it = ctx->tuned.find(tune_key(ctx->L, c));
+      if (it != ctx->tuned.end()) {
+        class_variant[c] = it->second;
+      } else if (ctx->autotune) {
+        // A sample of the class, evenly spaced.
+        const size_t want = std::min<size_t>(byclass[c].size(), 512);
+        std::vector<uint64_t> so, to;
+        std::vector<uint32_t> sn, tn;
+        for (size_t s = 0; s < want; ++s) {
+          const size_t i = byclass[c][s * byclass[c].size() / want];
+          so.push_back(u_off[i]);
+          sn.push_back(nu[i]);
+          to.push_back(v_off[i]);
+          tn.push_back(nv[i]);
+        }
+        int T, K, Bq;
+        rc = mgcd_autotune(ctx, limbs, so.data(), sn.data(), to.data(), tn.data(), want, &T, &K, &Bq);
+        if (rc == MGCD_OK) {
+          class_variant[c] = find_kernel_variant(T, K, Bq, ctx->wide);
+          ctx->tuned[tune_key(ctx->L, c)] = class_variant[c];
+          tuned_any = true;
+        }
+      } else {
+        class_variant[c] = default_variant(ctx, c);
+      }
+    }
+    if (tuned_any) save_tune_cache(ctx);
   }
-  const int vi = find_kernel_variant(T, K, Bp, ctx->wide);
-  int rc = vi < 0 ? MGCD_ERR_ARG : MGCD_OK;
-  if (rc == MGCD_OK && !b->pd.empty()) {
-    std::vector<uint32_t> order;
-    rc = plan_items(ctx, vi, b->pd, order, b->groups, b->items, b->ls);
-    std::vector<int64_t> pos(order.size());
-    for (size_t j = 0; j < order.size(); ++j) pos[order[j]] = (int64_t)j;
-    for (auto& k : b->kidx)
-      if (k >= 0) k = pos[k];
-  }
+  if (rc == MGCD_OK && !b->pd.empty()) rc = plan_batch(b, forced, class_variant);
   if (rc == MGCD_OK && !b->pd.empty()) {
     const size_t np = b->pd.size();
     if (b->limbs.ensure(packed.size() * 4 + 4) != hipSuccess || b->pairs.ensure(np * sizeof(PairDesc)) != hipSuccess ||
         b->groups_d.ensure(b->groups.size() * sizeof(GroupDesc)) != hipSuccess ||
-        b->items_d.ensure(b->items.size() * sizeof(WorkItem)) != hipSuccess || b->queue.ensure(kMaxQueues * kQueueStride * 4) != hipSuccess ||
-        b->bar.ensure((b->ls.nbar + 1) * 8) != hipSuccess || b->digits.ensure(b->ls.ndigits * 8) != hipSuccess ||
-        b->trace.ensure(b->ls.ntrace * 8) != hipSuccess || b->res.ensure(b->ls.nres * 4) != hipSuccess ||
+        b->items_d.ensure(b->items.size() * sizeof(WorkItem)) != hipSuccess ||
+        b->queue.ensure(b->launches.size() * kMaxQueues * kQueueStride * 4) != hipSuccess ||
+        b->bar.ensure((b->nbar + 1) * 8) != hipSuccess || b->digits.ensure(b->ndigits * 8) != hipSuccess ||
+        b->trace.ensure(b->ntrace * 8) != hipSuccess || b->res.ensure(b->nres * 4) != hipSuccess ||
         b->out.ensure(np * sizeof(PairOut)) != hipSuccess) {
       rc = MGCD_ERR_NOMEM;
     } else if (hipMemcpy(b->limbs.p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
@@ -743,6 +982,9 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
                    hipSuccess) {
       rc = MGCD_ERR_HIP;
     }
+    for (BatchLaunch& l : b->launches)
+      if (rc == MGCD_OK && (hipEventCreate(&l.e0) != hipSuccess || hipEventCreate(&l.e1) != hipSuccess))
+        rc = MGCD_ERR_HIP;
   }
   if (rc != MGCD_OK) {
     mgcd_batch_destroy(b);
@@ -760,31 +1002,47 @@ int mgcd_batch_run(mgcd_batch* b) {
   b->device_s = 0.0;
   b->host_s = 0.0;
   if (np) {
-    KernelArgs ka;
-    ka.limbs = b->limbs.as<uint32_t>();
-    ka.pairs = b->pairs.as<PairDesc>();
-    ka.groups = b->groups_d.as<GroupDesc>();
-    ka.items = b->items_d.as<WorkItem>();
-    ka.L = ctx->L;
-    ka.queue = b->queue.as<uint32_t>();
-    ka.mod_u32 = ctx->d_mod_u32.as<uint32_t>();
-    ka.mod_d = ctx->d_mod_d.as<double>();
-    ka.mod_inv = ctx->d_mod_inv.as<double>();
-    ka.bar = b->bar.as<unsigned long long>();
-    ka.digits = b->digits.as<mgcd_pair_t>();
-    ka.trace = b->trace.as<mgcd_pair_t>();
-    ka.res = b->res.as<uint32_t>();
-    ka.out = b->out.as<PairOut>();
-    ka.spin_limit = kSpinLimit;
-    int rc = launch(ctx, b->ls, ka);
-    if (rc != MGCD_OK) return rc;
+    hipStream_t s = ctx->stream;
+    if (b->use_graph) {
+      if (!b->graph_exec) {
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = MGCD_OK;
+        for (const BatchLaunch& l : b->launches)
+          if (rc == MGCD_OK) rc = enqueue_launch(b, l, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc != MGCD_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        HIP_TRY(e);
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+      }
+      HIP_TRY(hipEventRecord(ctx->ev0, s));
+      HIP_TRY(hipGraphLaunch(b->graph_exec, s));
+      HIP_TRY(hipEventRecord(ctx->ev1, s));
+    } else {
+      HIP_TRY(hipEventRecord(ctx->ev0, s));
+      for (const BatchLaunch& l : b->launches) {
+        HIP_TRY(hipEventRecord(l.e0, s));
+        const int rc = enqueue_launch(b, l, true);
+        if (rc != MGCD_OK) return rc;
+        HIP_TRY(hipEventRecord(l.e1, s));
+      }
+      HIP_TRY(hipEventRecord(ctx->ev1, s));
+    }
     b->h_out.resize(np);
-    b->h_digits.resize(b->ls.ndigits);
-    HIP_TRY(hipMemcpyAsync(b->h_out.data(), b->out.p, np * sizeof(PairOut), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(b->h_digits.data(), b->digits.p, b->ls.ndigits * 8, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    b->h_digits.resize(b->ndigits);
+    HIP_TRY(hipMemcpyAsync(b->h_out.data(), b->out.p, np * sizeof(PairOut), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(b->h_digits.data(), b->digits.p, b->ndigits * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) b->device_s = ms * 1e-3;
+    for (BatchLaunch& l : b->launches) {
+      l.device_s = 0.0;
+      if (!b->use_graph && hipEventElapsedTime(&ms, l.e0, l.e1) == hipSuccess) l.device_s = ms * 1e-3;
+    }
     ctx->last_kernel_s = b->device_s;
     // Digit conversion on host threads.
     const auto t0 = std::chrono::steady_clock::now();
@@ -816,25 +1074,29 @@ int mgcd_batch_run(mgcd_batch* b) {
     for (auto& x : th) x.join();
     b->host_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     // Pairs the reference's sizing left short of moduli: retry one by one
-    // with a quarter more moduli each time (MGCD_OPT_RETRY).
-    if (ctx->retry && !ctx->force_nq) {
-      for (size_t i : todo) {
-        if (b->status[i] != MGCD_ERR_REDUX && b->status[i] != MGCD_ERR_RECOVERY) continue;
-        const PairDesc& p = b->pd[b->kidx[i]];
-        const uint32_t* a = b->packed.data() + p.u_off;
-        const uint32_t* c = b->packed.data() + p.v_off;
-        const int table = (int)ctx->moduli.size();
-        int nq = (int)p.nq, rc = b->status[i];
-        std::vector<uint32_t> gl(p.nv + 1);
-        size_t ng = 0;
-        while ((rc == MGCD_ERR_REDUX || rc == MGCD_ERR_RECOVERY) && nq < table) {
-          nq = std::min(table, ((nq + nq / 4 + 255) / 256) * 256);
-          rc = gcd_single(ctx, a, p.nu, c, p.nv, nq, gl.data(), gl.size(), &ng, nullptr, 0, nullptr, nullptr, 0,
-                          nullptr);
-        }
-        b->status[i] = rc;
-        b->host_result[i].assign(gl.begin(), gl.begin() + (rc == MGCD_OK ? ng : 0));
+    // with a quarter more moduli each time (MGCD_OPT_RETRY).  A pair whose
+    // in-kernel wait timed out is run again on the one-pair path.
+    for (size_t i : todo) {
+      const int st = b->status[i];
+      const bool more = ctx->retry && !ctx->force_nq && (st == MGCD_ERR_REDUX || st == MGCD_ERR_RECOVERY);
+      if (!more && st != MGCD_ERR_TIMEOUT) continue;
+      const PairDesc& p = b->pd[b->kidx[i]];
+      const uint32_t* a = b->packed.data() + p.u_off;
+      const uint32_t* c = b->packed.data() + p.v_off;
+      const int table = (int)ctx->moduli.size();
+      int nq = (int)p.nq, rc = st;
+      std::vector<uint32_t> gl(p.nv + 1);
+      size_t ng = 0;
+      if (st == MGCD_ERR_TIMEOUT)
+        rc = gcd_single(ctx, a, p.nu, c, p.nv, nq, gl.data(), gl.size(), &ng, nullptr, 0, nullptr, nullptr, 0,
+                        nullptr);
+      while (ctx->retry && !ctx->force_nq && (rc == MGCD_ERR_REDUX || rc == MGCD_ERR_RECOVERY) && nq < table) {
+        nq = std::min(table, ((nq + nq / 4 + 255) / 256) * 256);
+        rc = gcd_single(ctx, a, p.nu, c, p.nv, nq, gl.data(), gl.size(), &ng, nullptr, 0, nullptr, nullptr, 0,
+                        nullptr);
       }
+      b->status[i] = rc;
+      b->host_result[i].assign(gl.begin(), gl.begin() + (rc == MGCD_OK ? ng : 0));
     }
   }
   b->ran = true;
@@ -883,11 +1145,35 @@ int mgcd_batch_pair_stats(const mgcd_batch* b, size_t i, uint32_t* nq, uint32_t*
 
 int mgcd_batch_tiling(const mgcd_batch* b, int* workgroup, int* slots, int* group) {
   if (!b) return MGCD_ERR_ARG;
-  const bool none = b->ls.variant < 0;
-  const KernelVariant* kv = none ? nullptr : &kernel_variant(b->ls.variant);
-  if (workgroup) *workgroup = none ? 0 : kv->T;
-  if (slots) *slots = none ? 0 : kv->K;
-  if (group) *group = none ? 0 : kv->B;
+  // The launch with the most pairs.
+  const BatchLaunch* best = nullptr;
+  for (const BatchLaunch& l : b->launches)
+    if (!best || l.pair_count > best->pair_count) best = &l;
+  const KernelVariant* kv = best ? &kernel_variant(best->variant) : nullptr;
+  if (workgroup) *workgroup = kv ? kv->T : 0;
+  if (slots) *slots = kv ? kv->K : 0;
+  if (group) *group = kv ? kv->B : 0;
+  return MGCD_OK;
+}
+
+int mgcd_batch_launch_count(const mgcd_batch* b, size_t* count) {
+  if (!b || !count) return MGCD_ERR_ARG;
+  *count = b->launches.size();
+  return MGCD_OK;
+}
+
+int mgcd_batch_launch_info(const mgcd_batch* b, size_t i, int* workgroup, int* slots, int* group, size_t* npairs,
+                           uint32_t* nq_min, uint32_t* nq_max, double* device_s) {
+  if (!b || i >= b->launches.size()) return MGCD_ERR_ARG;
+  const BatchLaunch& l = b->launches[i];
+  const KernelVariant& kv = kernel_variant(l.variant);
+  if (workgroup) *workgroup = kv.T;
+  if (slots) *slots = kv.K;
+  if (group) *group = kv.B;
+  if (npairs) *npairs = l.pair_count;
+  if (nq_min) *nq_min = l.nq_min;
+  if (nq_max) *nq_max = l.nq_max;
+  if (device_s) *device_s = l.device_s;
   return MGCD_OK;
 }
 
@@ -895,10 +1181,52 @@ int mgcd_batch_destroy(mgcd_batch* b) {
   if (!b) return MGCD_OK;
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (BatchLaunch& l : b->launches) {
+    if (l.e0) (void)hipEventDestroy(l.e0);
+    if (l.e1) (void)hipEventDestroy(l.e1);
+  }
   for (DevBuf* d : {&b->limbs, &b->pairs, &b->groups_d, &b->items_d, &b->queue, &b->bar, &b->digits, &b->trace, &b->res,
                     &b->out})
     d->release();
   delete b;
+  return MGCD_OK;
+}
+
+// Measures each candidate tiling on the given pairs (one warm-up and one
+// timed run of a batch with that tiling) and returns the fastest.
+int mgcd_autotune(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, const uint32_t* nu,
+                  const uint64_t* v_off, const uint32_t* nv, size_t npairs, int* workgroup, int* slots, int* group) {
+  if (!ctx || !workgroup || !slots || !group || npairs == 0) return MGCD_ERR_ARG;
+  double best_t = 0.0;
+  int best = -1;
+  const bool graph = ctx->batch_graph;
+  ctx->batch_graph = false;
+  for (const auto& c : kTuneCandidates) {
+    const int vi = find_kernel_variant(c[0], c[1], c[2], ctx->wide);
+    if (vi < 0) continue;
+    mgcd_batch* b = nullptr;
+    if (mgcd_batch_create(ctx, limbs, u_off, nu, v_off, nv, npairs, c[0], c[1], c[2], &b) != MGCD_OK) continue;
+    double t = 0.0;
+    bool ok = mgcd_batch_run(b) == MGCD_OK;
+    if (ok) ok = mgcd_batch_run(b) == MGCD_OK;
+    if (ok) {
+      t = b->device_s;
+      for (size_t i = 0; i < npairs; ++i) ok = ok && b->status[i] == MGCD_OK;
+    }
+    mgcd_batch_destroy(b);
+    if (ok && (best < 0 || t < best_t)) {
+      best_t = t;
+      best = vi;
+    }
+  }
+  ctx->batch_graph = graph;
+  if (best < 0) return MGCD_ERR_KERNEL;
+  const KernelVariant& kv = kernel_variant(best);
+  *workgroup = kv.T;
+  *slots = kv.K;
+  *group = kv.B;
   return MGCD_OK;
 }
 
@@ -934,6 +1262,392 @@ int mgcd_certify_quasi_quotient(mgcd_ctx* ctx, int mode, int L, uint64_t* failur
   rc = run();
   cnt.release();
   return rc;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Serving: a request loop that admits GCD requests of mixed sizes against a
+// device-memory budget, batches whatever is waiting into the next launch,
+// and keeps several launches in flight on their own streams so that work
+// admitted while others run overlaps with them on the device.
+
+namespace {
+
+struct ServerReq {
+  uint64_t ticket = 0;
+  std::vector<uint32_t> u, v;  // trimmed, |u| >= |v| > 0
+  uint32_t nq = 0;
+  uint64_t bytes = 0;          // device footprint while resident
+  std::chrono::steady_clock::time_point t_submit;
+  double latency = 0.0;
+  int status = MGCD_OK;
+  std::vector<uint32_t> result;
+  bool done = false;
+};
+
+// One launch slot: a stream, its device buffers and the requests in flight.
+struct ServerSlot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done_ev = nullptr;
+  DevBuf limbs, pairs, groups, items, queue, bar, digits, trace, res, out;
+  std::vector<ServerReq*> reqs;
+  std::vector<PairDesc> pd;
+  std::vector<uint32_t> order;
+  std::vector<PairOut> h_out;
+  std::vector<mgcd_pair_t> h_digits;
+  LaunchSet ls;
+  uint64_t bytes = 0;
+  bool busy = false;
+};
+
+// Device bytes a pair holds while resident: limbs, residues, pivots,
+// digits, descriptors.
+uint64_t footprint(size_t nu, size_t nv, uint32_t nq) {
+  return 4ull * (nu + nv) + 24ull * (nq + 64) + sizeof(PairDesc) + sizeof(PairOut) + sizeof(GroupDesc) +
+         32 * sizeof(WorkItem);
+}
+
+}  // namespace
+
+struct mgcd_server {
+  mgcd_ctx* ctx = nullptr;
+  uint64_t budget = 0, used = 0, peak = 0;
+  int variant = -1;
+  uint32_t grid_cap = 0;
+  std::vector<ServerSlot> slots;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<ServerReq*> pending;
+  std::unordered_map<uint64_t, ServerReq*> reqs;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  std::thread worker;
+  std::vector<double> latencies;
+  uint64_t completed = 0;
+  std::chrono::steady_clock::time_point t_first, t_last;
+  bool any = false;
+};
+
+namespace {
+
+// Launches slot `sl` with the requests in sl.reqs (worker thread, no lock held).
+int server_launch(mgcd_server* s, ServerSlot& sl) {
+  mgcd_ctx* ctx = s->ctx;
+  std::vector<uint32_t> packed;
+  sl.pd.clear();
+  for (ServerReq* r : sl.reqs) {
+    PairDesc p;
+    std::memset(&p, 0, sizeof p);
+    p.u_off = packed.size();
+    packed.insert(packed.end(), r->u.begin(), r->u.end());
+    p.v_off = packed.size();
+    packed.insert(packed.end(), r->v.begin(), r->v.end());
+    p.nu = (uint32_t)r->u.size();
+    p.nv = (uint32_t)r->v.size();
+    p.nq = r->nq;
+    sl.pd.push_back(p);
+  }
+  std::vector<GroupDesc> groups;
+  std::vector<WorkItem> items;
+  sl.ls = LaunchSet();
+  int rc = plan_items(ctx, s->variant, sl.pd, sl.order, groups, items, sl.ls);
+  if (rc != MGCD_OK) return rc;
+  // Every slot's grid fits the device at the same time as the others'.
+  sl.ls.grid = std::min(sl.ls.grid, s->grid_cap);
+  uint32_t maxG = 0;
+  for (const GroupDesc& g : groups) maxG = std::max(maxG, g.G);
+  if (maxG > sl.ls.grid) return MGCD_ERR_TOO_MANY_THREADS;
+  if (sl.ls.nqueues > 1 && maxG > sl.ls.grid / 2) {  // keep the queue plan's no-stall condition
+    sl.ls.nqueues = 1;
+    sl.ls.qoff[0] = 0;
+    sl.ls.qoff[1] = (uint32_t)items.size();
+  }
+  const size_t np = sl.pd.size();
+  if (sl.limbs.ensure(packed.size() * 4 + 4) != hipSuccess || sl.pairs.ensure(np * sizeof(PairDesc)) != hipSuccess ||
+      sl.groups.ensure(groups.size() * sizeof(GroupDesc)) != hipSuccess ||
+      sl.items.ensure(items.size() * sizeof(WorkItem)) != hipSuccess ||
+      sl.queue.ensure(kMaxQueues * kQueueStride * 4) != hipSuccess || sl.bar.ensure((sl.ls.nbar + 1) * 8) != hipSuccess ||
+      sl.digits.ensure(sl.ls.ndigits * 8) != hipSuccess || sl.trace.ensure(sl.ls.ntrace * 8) != hipSuccess ||
+      sl.res.ensure(sl.ls.nres * 4) != hipSuccess || sl.out.ensure(np * sizeof(PairOut)) != hipSuccess)
+    return MGCD_ERR_NOMEM;
+  hipStream_t st = sl.stream;
+  HIP_TRY(hipMemcpyAsync(sl.limbs.p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(sl.pairs.p, sl.pd.data(), np * sizeof(PairDesc), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(sl.groups.p, groups.data(), groups.size() * sizeof(GroupDesc), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(sl.items.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st));
+  KernelArgs ka;
+  std::memset(&ka, 0, sizeof ka);
+  ka.limbs = sl.limbs.as<uint32_t>();
+  ka.pairs = sl.pairs.as<PairDesc>();
+  ka.groups = sl.groups.as<GroupDesc>();
+  ka.items = sl.items.as<WorkItem>();
+  ka.L = ctx->L;
+  ka.queue = sl.queue.as<uint32_t>();
+  ka.mod_u32 = ctx->d_mod_u32.as<uint32_t>();
+  ka.mod_d = ctx->d_mod_d.as<double>();
+  ka.mod_inv = ctx->d_mod_inv.as<double>();
+  ka.bar = sl.bar.as<unsigned long long>();
+  ka.digits = sl.digits.as<mgcd_pair_t>();
+  ka.trace = sl.trace.as<mgcd_pair_t>();
+  ka.res = sl.res.as<uint32_t>();
+  ka.out = sl.out.as<PairOut>();
+  ka.spin_limit = kSpinLimit;
+  ka.nqueues = sl.ls.nqueues;
+  std::memcpy(ka.qoff, sl.ls.qoff, sizeof ka.qoff);
+  HIP_TRY(launch_fill_u32(ka.queue, kMaxQueues * kQueueStride, 0u, st));
+  if (sl.ls.nbar) HIP_TRY(launch_fill_u64(ka.bar, sl.ls.nbar, 0ull, st));
+  // Plain launch: the slots' grids together fit the device, so all of their
+  // workgroups are resident at once while slots overlap.
+  const KernelVariant& kv = kernel_variant(s->variant);
+  void* params[] = {&ka};
+  HIP_TRY(launch_resident(kv.fn, sl.ls.grid, kv.T, params, st, false));
+  sl.h_out.resize(np);
+  sl.h_digits.resize(sl.ls.ndigits);
+  HIP_TRY(hipMemcpyAsync(sl.h_out.data(), sl.out.p, np * sizeof(PairOut), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(sl.h_digits.data(), sl.digits.p, sl.ls.ndigits * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipEventRecord(sl.done_ev, st));
+  return MGCD_OK;
+}
+
+// Results of a finished slot (worker thread, no lock held).  Pairs the
+// batch could not finish (moduli short, a wait timed out) go through the
+// one-pair path with the retry policy.
+void server_finish(mgcd_server* s, ServerSlot& sl, int launch_rc) {
+  mgcd_ctx* ctx = s->ctx;
+  for (size_t j = 0; j < sl.pd.size(); ++j) {
+    ServerReq* r = sl.reqs[sl.order[j]];
+    const PairDesc& p = sl.pd[j];
+    int st = launch_rc;
+    if (st == MGCD_OK) {
+      const PairOut& o = sl.h_out[j];
+      st = o.status != KST_OK ? kst_to_status(o.status) : (o.ndig > p.nq ? MGCD_ERR_KERNEL : MGCD_OK);
+      if (st == MGCD_OK) digits_to_limbs(sl.h_digits.data() + p.dig_off, o.ndig, r->result);
+    }
+    if (st == MGCD_ERR_REDUX || st == MGCD_ERR_RECOVERY || st == MGCD_ERR_TIMEOUT) {
+      std::vector<uint32_t> g(r->v.size() + 1);
+      size_t ng = 0;
+      st = mgcd_gcd_limbs(ctx, r->u.data(), r->u.size(), r->v.data(), r->v.size(), g.data(), g.size(), &ng);
+      if (st == MGCD_OK) r->result.assign(g.begin(), g.begin() + ng);
+    }
+    r->status = st;
+  }
+}
+
+void server_loop(mgcd_server* s) {
+  (void)hipSetDevice(s->ctx->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    bool progress = false;
+    // Completions.
+    for (ServerSlot& sl : s->slots) {
+      if (!sl.busy || hipEventQuery(sl.done_ev) != hipSuccess) continue;
+      lk.unlock();
+      server_finish(s, sl, MGCD_OK);
+      lk.lock();
+      const auto now = std::chrono::steady_clock::now();
+      for (ServerReq* r : sl.reqs) {
+        r->latency = std::chrono::duration<double>(now - r->t_submit).count();
+        r->done = true;
+        s->latencies.push_back(r->latency);
+        s->completed++;
+      }
+      s->t_last = now;
+      s->used -= sl.bytes;
+      sl.bytes = 0;
+      sl.reqs.clear();
+      sl.busy = false;
+      progress = true;
+      s->cv.notify_all();
+    }
+    // Admission: waiting requests in arrival order, while they fit the
+    // budget, into a free slot.
+    for (ServerSlot& sl : s->slots) {
+      if (sl.busy || s->pending.empty()) continue;
+      while (!s->pending.empty() && s->used + sl.bytes + s->pending.front()->bytes <= s->budget &&
+             sl.reqs.size() < 8192) {
+        ServerReq* r = s->pending.front();
+        s->pending.pop_front();
+        sl.reqs.push_back(r);
+        sl.bytes += r->bytes;
+      }
+      if (sl.reqs.empty()) break;  // the head waits for memory
+      s->used += sl.bytes;
+      s->peak = std::max(s->peak, s->used);
+      sl.busy = true;
+      lk.unlock();
+      const int rc = server_launch(s, sl);
+      lk.lock();
+      if (rc != MGCD_OK) {  // nothing is in flight on this slot: answer now
+        (void)hipStreamSynchronize(sl.stream);
+        lk.unlock();
+        server_finish(s, sl, rc);
+        lk.lock();
+        const auto now = std::chrono::steady_clock::now();
+        for (ServerReq* r : sl.reqs) {
+          r->latency = std::chrono::duration<double>(now - r->t_submit).count();
+          r->done = true;
+          s->completed++;
+        }
+        s->used -= sl.bytes;
+        sl.bytes = 0;
+        sl.reqs.clear();
+        sl.busy = false;
+        s->cv.notify_all();
+      }
+      progress = true;
+    }
+    bool busy = false;
+    for (const ServerSlot& sl : s->slots) busy |= sl.busy;
+    if (s->stop && !busy && s->pending.empty()) break;
+    if (!progress) {
+      if (busy)
+        s->cv.wait_for(lk, std::chrono::microseconds(100));
+      else
+        s->cv.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgcd_server_create(mgcd_ctx* ctx, uint64_t budget_bytes, int slots, mgcd_server** out) {
+  if (!ctx || !out || budget_bytes == 0 || slots < 1 || slots > 8) return MGCD_ERR_ARG;
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  mgcd_server* s = new mgcd_server();
+  s->ctx = ctx;
+  s->budget = budget_bytes;
+  s->variant = find_kernel_variant(256, 2, 16, ctx->wide);
+  if (s->variant < 0 || ctx->max_resident[s->variant] < slots) {
+    delete s;
+    return MGCD_ERR_TOO_MANY_THREADS;
+  }
+  s->grid_cap = (uint32_t)(ctx->max_resident[s->variant] / slots);
+  s->slots.resize(slots);
+  for (ServerSlot& sl : s->slots) {
+    if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.done_ev, hipEventDisableTiming) != hipSuccess) {
+      mgcd_server_destroy(s);
+      return MGCD_ERR_HIP;
+    }
+  }
+  s->worker = std::thread(server_loop, s);
+  *out = s;
+  return MGCD_OK;
+}
+
+int mgcd_server_submit(mgcd_server* s, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint64_t* ticket) {
+  if (!s || !ticket || (nu && !u) || (nv && !v)) return MGCD_ERR_ARG;
+  nu = trim(u, nu);
+  nv = trim(v, nv);
+  if (cmp_mag(u, nu, v, nv) < 0) {
+    std::swap(u, v);
+    std::swap(nu, nv);
+  }
+  ServerReq* r = new ServerReq();
+  r->t_submit = std::chrono::steady_clock::now();
+  r->u.assign(u, u + nu);
+  r->v.assign(v, v + nv);
+  bool immediate = false;
+  if (nv == 0) {  // gcd(u, 0) = |u|, answered on the host
+    r->result = r->u;
+    immediate = true;
+  } else {
+    const int nqi = mgcd_num_moduli_needed(std::max<uint64_t>(bit_length(u, nu), 2), s->ctx->L);
+    if (nqi <= 0 || (size_t)nqi > s->ctx->moduli.size()) {
+      r->status = MGCD_ERR_NOT_ENOUGH_MODULI;
+      immediate = true;
+    } else {
+      r->nq = (uint32_t)nqi;
+      r->bytes = footprint(nu, nv, r->nq);
+      if (r->bytes > s->budget) {
+        delete r;
+        return MGCD_ERR_BUSY;
+      }
+    }
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  r->ticket = s->next_ticket++;
+  *ticket = r->ticket;
+  s->reqs[r->ticket] = r;
+  if (!s->any) {
+    s->any = true;
+    s->t_first = r->t_submit;
+  }
+  if (immediate) {
+    r->done = true;
+    s->completed++;
+    s->latencies.push_back(0.0);
+  } else {
+    s->pending.push_back(r);
+  }
+  s->cv.notify_all();
+  return MGCD_OK;
+}
+
+int mgcd_server_wait(mgcd_server* s, uint64_t ticket, double timeout_s, uint32_t* g, size_t cap, size_t* ng,
+                     double* latency_s) {
+  if (!s || !ng) return MGCD_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return MGCD_ERR_ARG;
+  ServerReq* r = it->second;
+  const auto pred = [&] { return r->done; };
+  if (timeout_s < 0)
+    s->cv.wait(lk, pred);
+  else if (!s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
+    return MGCD_ERR_TIMEOUT;
+  if (latency_s) *latency_s = r->latency;
+  *ng = r->result.size();
+  const int st = r->status;
+  if (st == MGCD_OK && r->result.size() > cap) return MGCD_ERR_CAPACITY;  // ticket kept: ask again
+  if (st == MGCD_OK && !r->result.empty()) std::memcpy(g, r->result.data(), r->result.size() * 4);
+  s->reqs.erase(it);
+  delete r;
+  return st;
+}
+
+int mgcd_server_stats(mgcd_server* s, uint64_t* completed, double* p50_s, double* p99_s, double* gcds_per_s,
+                      uint64_t* peak_bytes) {
+  if (!s) return MGCD_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::vector<double> l = s->latencies;
+  std::sort(l.begin(), l.end());
+  auto q = [&](double f) { return l.empty() ? 0.0 : l[std::min(l.size() - 1, (size_t)std::ceil(f * l.size()) - 1)]; };
+  if (completed) *completed = s->completed;
+  if (p50_s) *p50_s = q(0.50);
+  if (p99_s) *p99_s = q(0.99);
+  if (gcds_per_s) {
+    const double span = s->any ? std::chrono::duration<double>(s->t_last - s->t_first).count() : 0.0;
+    *gcds_per_s = span > 0 ? s->completed / span : 0.0;
+  }
+  if (peak_bytes) *peak_bytes = s->peak;
+  return MGCD_OK;
+}
+
+int mgcd_server_destroy(mgcd_server* s) {
+  if (!s) return MGCD_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv.notify_all();
+  }
+  if (s->worker.joinable()) s->worker.join();
+  (void)hipSetDevice(s->ctx->device);
+  for (ServerSlot& sl : s->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    for (DevBuf* d : {&sl.limbs, &sl.pairs, &sl.groups, &sl.items, &sl.queue, &sl.bar, &sl.digits, &sl.trace, &sl.res,
+                      &sl.out})
+      d->release();
+    if (sl.done_ev) (void)hipEventDestroy(sl.done_ev);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  for (auto& kv : s->reqs) delete kv.second;
+  delete s;
+  return MGCD_OK;
 }
 
 }  // extern "C"

@@ -36,13 +36,17 @@ EXPORTS = [
     "mgcd_ctx_moduli", "mgcd_gcd_limbs", "mgcd_gcd_limbs_trace", "mgcd_batch_create", "mgcd_batch_run",
     "mgcd_batch_status", "mgcd_batch_result", "mgcd_batch_timing", "mgcd_batch_tiling", "mgcd_batch_destroy",
     "mgcd_certify_quasi_quotient", "mgcd_last_error_detail", "mgcd_ctx_last_kernel_seconds",
-    "mgcd_ctx_synchronize", "mgcd_batch_pair_stats", "mgcd_ctx_set_option",
+    "mgcd_ctx_synchronize", "mgcd_batch_pair_stats", "mgcd_ctx_set_option", "mgcd_batch_launch_count",
+    "mgcd_batch_launch_info", "mgcd_autotune", "mgcd_server_create", "mgcd_server_submit", "mgcd_server_wait",
+    "mgcd_server_stats", "mgcd_server_destroy",
 ]
 
 OPT_FORCE_MODULI = 1
 OPT_RETRY = 2
 OPT_SINGLE_WORKGROUP = 3
 OPT_SINGLE_SLOTS = 4
+OPT_AUTOTUNE = 5
+OPT_BATCH_GRAPH = 6
 
 
 class Pair(ctypes.Structure):
@@ -102,6 +106,17 @@ def lib():
         L.mgcd_ctx_synchronize.argtypes = [ctypes.c_void_p]
         L.mgcd_ctx_set_option.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]
         L.mgcd_batch_pair_stats.argtypes = [ctypes.c_void_p, ctypes.c_size_t, P32, P32, P32]
+        L.mgcd_batch_launch_count.argtypes = [ctypes.c_void_p, PSZ]
+        L.mgcd_batch_launch_info.argtypes = [ctypes.c_void_p, ctypes.c_size_t, PI, PI, PI, PSZ, P32, P32,
+                                             ctypes.POINTER(ctypes.c_double)]
+        L.mgcd_autotune.argtypes = [ctypes.c_void_p, P32, P64, P32, P64, P32, ctypes.c_size_t, PI, PI, PI]
+        L.mgcd_server_create.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.mgcd_server_submit.argtypes = [ctypes.c_void_p, P32, ctypes.c_size_t, P32, ctypes.c_size_t, P64]
+        L.mgcd_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double, P32, ctypes.c_size_t, PSZ,
+                                       ctypes.POINTER(ctypes.c_double)]
+        L.mgcd_server_stats.argtypes = [ctypes.c_void_p, P64, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), P64]
+        L.mgcd_server_destroy.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -238,6 +253,16 @@ def pack_pairs(pairs):
             np.array(v_off, np.uint64), np.array(nv, np.uint32))
 
 
+def autotune(ctx, pairs):
+    """The fastest batch tiling (threads, slots, pairs per group) measured on these pairs."""
+    limbs, u_off, nu, v_off, nv = pack_pairs(pairs)
+    t, k, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(lib().mgcd_autotune(ctx.h, _p32(limbs), u_off.ctypes.data_as(P64), _p32(nu), v_off.ctypes.data_as(P64),
+                              _p32(nv), len(nu), ctypes.byref(t), ctypes.byref(k), ctypes.byref(g)),
+          "mgcd_autotune")
+    return t.value, k.value, g.value
+
+
 class Batch:
     """Pairs resident in HBM, computed by the persistent kernel."""
 
@@ -289,9 +314,71 @@ class Batch:
         check(lib().mgcd_batch_tiling(self.h, ctypes.byref(t), ctypes.byref(k), ctypes.byref(g)))
         return t.value, k.value, g.value
 
+    def launches(self):
+        """Per launch: dict(tiling, pairs, nq range, device seconds of the last run)."""
+        n = ctypes.c_size_t()
+        check(lib().mgcd_batch_launch_count(self.h, ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            t, k, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            npairs, lo, hi, sec = ctypes.c_size_t(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+            check(lib().mgcd_batch_launch_info(self.h, i, ctypes.byref(t), ctypes.byref(k), ctypes.byref(g),
+                                               ctypes.byref(npairs), ctypes.byref(lo), ctypes.byref(hi),
+                                               ctypes.byref(sec)))
+            out.append({"tiling": (t.value, k.value, g.value), "pairs": npairs.value, "nq": (lo.value, hi.value),
+                        "device_s": sec.value})
+        return out
+
     def close(self):
         if self.h:
             lib().mgcd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop (mgcd_server_*): submit() returns a
+    ticket at once; wait(ticket) returns (gcd, latency seconds)."""
+
+    def __init__(self, ctx, budget_bytes=64 << 30, slots=2):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        check(lib().mgcd_server_create(ctx.h, budget_bytes, slots, ctypes.byref(h)), "mgcd_server_create")
+        self.h = h
+        self._cap = {}
+
+    def submit(self, u, v):
+        ua, va = int_to_limbs(abs(u)), int_to_limbs(abs(v))
+        t = ctypes.c_uint64()
+        check(lib().mgcd_server_submit(self.h, _p32(ua), len(ua), _p32(va), len(va), ctypes.byref(t)),
+              "mgcd_server_submit")
+        self._cap[t.value] = max(len(ua), len(va)) + 1
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        cap = self._cap.pop(ticket)
+        g = np.zeros(cap, np.uint32)
+        ng, lat = ctypes.c_size_t(), ctypes.c_double()
+        check(lib().mgcd_server_wait(self.h, ticket, timeout, _p32(g), cap, ctypes.byref(ng), ctypes.byref(lat)),
+              "mgcd_server_wait")
+        return limbs_to_int(g[:ng.value]), lat.value
+
+    def stats(self):
+        n, p50, p99, rate, peak = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), \
+            ctypes.c_uint64()
+        check(lib().mgcd_server_stats(self.h, ctypes.byref(n), ctypes.byref(p50), ctypes.byref(p99),
+                                      ctypes.byref(rate), ctypes.byref(peak)))
+        return {"completed": n.value, "p50_s": p50.value, "p99_s": p99.value, "gcds_per_s": rate.value,
+                "peak_bytes": peak.value}
+
+    def close(self):
+        if self.h:
+            lib().mgcd_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -12,6 +12,7 @@ static int failures = 0;
 
 static void check(bool ok, const char* what) {
   std::printf("%s %s\n", ok ? "PASS" : "FAIL", what);
+  std::fflush(stdout);
   if (!ok) ++failures;
 }
 
@@ -49,6 +50,8 @@ int main() {
   // u = g a, v = g b at several sizes.
   bool all = true;
   for (unsigned bits : {100u, 1000u, 8192u, 30000u, 100000u}) {
+    std::printf("  gcd of %u-bit pair\n", bits);
+    std::fflush(stdout);
     mpz_urandomb(g, st, bits / 8 + 1);
     mpz_urandomb(u, st, bits);
     mpz_urandomb(v, st, bits);
