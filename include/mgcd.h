@@ -100,7 +100,12 @@ enum {
   /* 1: mgcd_batch_run captures the batch's launch sequence into a hipGraph
    * on its first run and replays the graph afterwards.  0 (default):
    * cooperative launches. */
-  MGCD_OPT_BATCH_GRAPH = 6
+  MGCD_OPT_BATCH_GRAPH = 6,
+  /* Pairs split over at least this many workgroups (default 48, >= 2)
+   * exchange their per-iteration minima through part 0 (two hops, traffic
+   * linear in the parts) instead of every part reading every part's slot
+   * (one hop, traffic quadratic in the parts). */
+  MGCD_OPT_RELAY_PARTS = 7
 };
 int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value);
 /* Blocks until all work queued on the context's device has finished. */
@@ -149,8 +154,9 @@ int mgcd_batch_pair_stats(const mgcd_batch* b, size_t i, uint32_t* nq, uint32_t*
  * moduli slots per thread, pairs per group. */
 int mgcd_batch_tiling(const mgcd_batch* b, int* workgroup, int* slots, int* group);
 /* A batch created with tiling (0, 0, 0) is split into size classes of N_Q
- * (up to 1024, 4096, 16384, more), one kernel launch each with its own
- * tiling; an explicit tiling gives one launch.  Launch i: tiling, pairs,
+ * (up to 1024, 4096, 16384, more), each with its own tiling; adjacent
+ * classes with the same tiling share a kernel launch.  An explicit tiling
+ * gives one launch.  Launch i: tiling, pairs,
  * N_Q range and the seconds its kernel took in the last run (0 when the run
  * replayed a graph). */
 int mgcd_batch_launch_count(const mgcd_batch* b, size_t* count);
@@ -172,8 +178,11 @@ int mgcd_autotune(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, c
  * persistent kernel, and keeps up to `slots` launches in flight on their
  * own streams (each with a 1/slots share of the device's workgroups), so
  * requests that arrive while others run are started as soon as a slot and
- * memory free up and overlap with them on the device.  The context belongs
- * to the server's worker until mgcd_server_destroy.  No counterpart in the
+ * memory free up and overlap with them on the device.  With two or more
+ * slots the last one takes only requests of more than 16,384 moduli (above
+ * ~200 kbit at L = 32), so small requests never wait behind a large one in
+ * the same launch.  The context belongs to the server's worker until
+ * mgcd_server_destroy.  No counterpart in the
  * reference (one blocking call per pair, GmpCudaDevice-gcd.cu:57-132).
  *   submit: copies the operands; MGCD_ERR_BUSY if the request alone
  *           exceeds the budget.

@@ -32,7 +32,7 @@ struct GroupDesc {
   uint32_t count;
   uint32_t nq;         // N_Q of every pair of the group
   uint32_t G;          // workgroups the group is split over
-  uint64_t bar_off;    // first barrier slot (2*G*B uint64 slots), if G > 1
+  uint64_t bar_off;    // first barrier slot (2*(G+1)*B uint64 slots), if G > 1
 };
 
 // One unit of the persistent work queue: part `part` of group `group`.
@@ -76,6 +76,7 @@ struct KernelArgs {
   uint32_t* res;               // residue scratch (2 N_Q per pair)
   PairOut* out;
   uint32_t spin_limit;         // polls before a barrier wait gives up
+  uint32_t relay_min_g;        // groups of at least this many parts exchange through part 0
 };
 
 // Kernel variants: threads per workgroup T, moduli slots per thread K,

@@ -223,7 +223,7 @@ struct Shared {
   int32_t status[B];
   uint32_t ntrace[B];
   double sel[B];              // recovery: the owner's (index, value) key
-  uint16_t surv[T * K];       // survivors of one pair (local offsets)
+  uint16_t surv[B * T * K];   // survivors of the group: pair << 11 | local offset
   uint32_t retire[T * K];     // bit b: this part's modulus is pair b's pivot
   uint32_t item;
 };
@@ -231,39 +231,76 @@ struct Shared {
 struct Ctx {
   uint32_t G, g;             // parts of the group, this part
   uint32_t rnd;              // cross-workgroup round counter
-  unsigned long long* bar;   // this group's 2*G*B slots
+  unsigned long long* bar;   // this group's 2*(G+1)*B slots
   uint32_t spin_limit;
+  bool relay;                // exchange through part 0 (large G)
 };
 
 // Cross-workgroup minima, called by wave 0 (all 64 lanes): lane b < B
-// brings this workgroup's value for pair b and gets the group's minimum;
-// KNONE for every pair when a wait timed out.  Slot layout of a row:
-// [G][B], so lane l polls only slots of pair l % B.
+// brings this workgroup's value for pair b; lane l gets the group's minimum
+// for pair l % B, KNONE for every pair when a wait timed out.  Slot layout
+// of a row: [G][B], so lane l polls only slots of pair l % B.  Every part
+// polls the whole row (one hop), or, for groups of many parts, part 0 polls
+// it and posts the minima into a relay row of B slots that the others poll
+// (two hops, but G*B + (G-1)*B polled slots per round instead of G*G*B).
 template <int B>
 __device__ double grid_min(double x, Ctx& C) {
   const uint32_t lane = __lane_id();
   C.rnd += 1;
   const uint64_t tag = (uint64_t)((C.rnd - 1) % 4095u + 1u);
   unsigned long long* row = C.bar + (size_t)(C.rnd & 1) * C.G * B;
+  unsigned long long* relay = C.bar + (size_t)2 * C.G * B + (size_t)(C.rnd & 1) * B;
   if (lane < (uint32_t)B)
     __hip_atomic_store(row + (size_t)C.g * B + lane, (unsigned long long)((tag << 52) | key_bits(x)),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (C.relay && C.g != 0) {
+    uint64_t v = __hip_atomic_load(relay + lane % B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t spins = 0;
+    bool late = false;
+    while ((v >> 52) != tag) {
+      if (++spins > C.spin_limit) {
+        late = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      v = __hip_atomic_load(relay + lane % B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // A relayed KNONE (part 0 timed out) arrives as the all-ones payload.
+    const bool none = (v & PAYLOAD) == PAYLOAD;
+    return (__ballot(late) || __ballot(none)) ? KNONE : key_from_bits(v & PAYLOAD);
+  }
   double m = KNONE;
   bool late = false;
   const uint32_t nslots = C.G * B;
-  for (uint32_t j = lane; j < nslots; j += 64) {
-    uint64_t v;
-    uint32_t spins = 0;
-    for (;;) {
-      v = __hip_atomic_load(row + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if ((v >> 52) == tag) break;
-      if (++spins > C.spin_limit) { late = true; break; }
-      __builtin_amdgcn_s_sleep(1);
+  // Four slots per lane per pass: their loads are in flight together, and
+  // only slots not yet posted are polled again.
+  for (uint32_t j0 = lane; j0 < nslots; j0 += 4 * 64) {
+    uint64_t v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t j = j0 + 64 * q;
+      v[q] = j < nslots ? __hip_atomic_load(row + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag << 52) | PAYLOAD;
     }
-    m = __builtin_fmin(m, key_from_bits(v & PAYLOAD));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t spins = 0;
+      while ((v[q] >> 52) != tag) {
+        if (++spins > C.spin_limit) {
+          late = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v[q] = __hip_atomic_load(row + j0 + 64 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      m = __builtin_fmin(m, key_from_bits(v[q] & PAYLOAD));
+    }
   }
   m = class_min<B>(m);
-  return __ballot(late) ? KNONE : m;
+  const bool any_late = __ballot(late) != 0;
+  if (C.relay && lane < (uint32_t)B)
+    __hip_atomic_store(relay + lane, (unsigned long long)((tag << 52) | (any_late ? PAYLOAD : key_bits(m))),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return any_late ? KNONE : m;
 }
 
 // (a) Residues of NOP operands at once (x[o], n[o] limbs) for every slot
@@ -529,10 +566,13 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         const double d = tv - load_after(S.bb, b, b == 0 ? tv : P[b - 1]);
         const bool fin = __builtin_fabs(tv) < TFIN;
         const bool ok = fin & (d != 0.0) & (__builtin_fabs(d) != M[k].m);
-        const double alt = tv == TINF ? 0.0 : (fin ? TINF : tv);
+        // Every value alt can take (0, TINF and the sentinels TZERO, TRET a
+        // non-finite t holds) has a zero low word: the selects work on the
+        // high words only.
+        const uint32_t alt_hi = tv == TINF ? 0u : (fin ? hi32(TINF) : hi32(tv));
         const double dz = d + ZOK;
-        const double y = ok ? d : 1.0;
-        t[b][k] = ok ? dz : alt;
+        const double y = mk64(ok ? lo32(d) : 0u, ok ? hi32(d) : hi32(1.0));
+        t[b][k] = mk64(ok ? lo32(dz) : 0u, ok ? hi32(dz) : alt_hi);
         P[b] = b == 0 ? y : A::mulmod(P[b - 1], y, M[k]);
       }
       double r = A::modinv(canon(P[B - 1], M[k]), M[k]);
@@ -542,7 +582,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         const bool ok = z > 0.5 * ZOK;
         const double zy = z - ZOK;
         const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
-        if (b > 0) r = A::mulmod(r, ok ? zy : 1.0, M[k]);
+        if (b > 0) r = A::mulmod(r, mk64(ok ? lo32(zy) : 0u, ok ? hi32(zy) : hi32(1.0)), M[k]);
         const double tn = tsigned(A::mulmod(load_after(S.pp, b, inv), inv, M[k]), M[k]);
         const double tv2 = ok ? tn : z;
         t[b][k] = tv2;
@@ -606,79 +646,100 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   __syncthreads();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
-  // ---- (c1) replay of the pivots on the survivors, pair by pair ----------
-  for (uint32_t b = 0; b < npairs; ++b) {
-    if (S.status[b] != KST_OK) continue;
-    const PairDesc& P = args.pairs[Gd.first + b];
-    const uint32_t R = S.ntrace[b];
-    // Compaction: survivor n (in slot-major, wave, lane order) -> S.surv[n].
-    uint32_t nsurv = 0;
+  // ---- (c1) replay of the pivots on the survivors -------------------------
+  // The survivors of every pair of the group form one compacted list
+  // (pair-major; entry = pair << 11 | slot offset); each lane replays up to
+  // four entries at a time, which may belong to different pairs: the pivots
+  // of all pairs are staged through LDS together, RC steps at a time.
+  {
+    uint32_t nsurv = 0, maxR = 0;
+    for (uint32_t b = 0; b < npairs; ++b) {
+      if (S.status[b] != KST_OK) continue;
+      maxR = max(maxR, S.ntrace[b]);
 #pragma unroll
-    for (int k = 0; k < K; ++k)
+      for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int w = 0; w < NW; ++w) {
-        const uint64_t word = S.actw[b][k][w];
-        if ((uint32_t)w == wave && ((word >> lane) & 1))
-          S.surv[nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(word >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)word, 0))] =
-              (uint16_t)(k * T + tid);
-        nsurv += (uint32_t)__builtin_popcountll(word);
-      }
-    __syncthreads();
-    ModD RM[K];
-    double ra[K], rc[K], rs[K];
-    uint64_t ro[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint32_t j = (uint32_t)k * T + tid;
-      RM[k].m = 3.0;
-      RM[k].minv = 1.0 / 3.0;
-      ra[k] = rc[k] = 0.0;
-      rs[k] = 1.0;
-      ro[k] = 0;
-      if (j < nsurv) {
-        const uint32_t idx = base + S.surv[j];
-        RM[k].m = args.mod_d[idx];
-        RM[k].minv = args.mod_inv[idx];
-        ro[k] = P.res_off + idx;
-        ra[k] = (double)args.res[ro[k]];
-        rc[k] = (double)args.res[ro[k] + nq];
-      }
+        for (int w = 0; w < NW; ++w) {
+          const uint64_t word = S.actw[b][k][w];
+          if ((uint32_t)w == wave && ((word >> lane) & 1))
+            S.surv[nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(word >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)word, 0))] =
+                (uint16_t)((b << 11) | (uint32_t)(k * T + tid));
+          nsurv += (uint32_t)__builtin_popcountll(word);
+        }
     }
-    // Slots in use by this wave (uniform).
-    int kw = 0;
+    __syncthreads();
+    constexpr int E = B >= 4 ? 4 : K;  // entries per lane per round
+    constexpr int RC = TILE / (2 * B);  // pivots per pair per staging
+    for (uint32_t e0 = 0; e0 < nsurv; e0 += E * T) {
+      ModD RM[E];
+      double ra[E], rc[E], rs[E];
+      uint32_t eb[E], eR[E];
+      uint64_t ro[E];
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-      if ((uint32_t)k * T + wave * 64 < nsurv) kw = k + 1;
-    for (uint32_t c0 = 0; c0 < R; c0 += TILE / 2) {
-      const uint32_t len = min((uint32_t)(TILE / 2), R - c0);
-      __syncthreads();
-      for (uint32_t j = tid; j < len; j += T) {
-        const mgcd_pair_t e = args.trace[P.trace_off + c0 + j];
-        S.tile[2 * j] = (double)e.value;
-        S.tile[2 * j + 1] = (double)e.modulus;
+      for (int e = 0; e < E; ++e) {
+        const uint32_t j = e0 + (uint32_t)e * T + tid;
+        RM[e].m = 3.0;
+        RM[e].minv = 1.0 / 3.0;
+        ra[e] = rc[e] = 0.0;
+        rs[e] = 1.0;
+        ro[e] = 0;
+        eb[e] = 0;
+        eR[e] = 0;
+        if (j < nsurv) {
+          const uint32_t ent = S.surv[j];
+          const uint32_t idx = base + (ent & 0x7FFu);
+          eb[e] = ent >> 11;
+          eR[e] = S.ntrace[eb[e]];
+          RM[e].m = args.mod_d[idx];
+          RM[e].minv = args.mod_inv[idx];
+          ro[e] = args.pairs[Gd.first + eb[e]].res_off + idx;
+          ra[e] = (double)args.res[ro[e]];
+          rc[e] = (double)args.res[ro[e] + nq];
+        }
       }
-      __syncthreads();
-      if (kw > 0) {
-        for (uint32_t j = 0; j < len; ++j) {
-          const double bv = S.tile[2 * j], pv = S.tile[2 * j + 1];
+      // Entries in use by this wave (uniform).
+      int ew = 0;
 #pragma unroll
-          for (int k = 0; k < K; ++k) {
-            if (k < kw) {
-              const double c2 = ra[k] - A::mulmod(bv, rc[k], RM[k]);  // |c2| <= m + 2^14
-              ra[k] = A::mulmod(rc[k], pv, RM[k]);
-              rs[k] = A::mulmod(rs[k], pv, RM[k]);
-              rc[k] = c2;
+      for (int e = 0; e < E; ++e)
+        if (e0 + (uint32_t)e * T + wave * 64 < nsurv) ew = e + 1;
+      for (uint32_t c0 = 0; c0 < maxR; c0 += RC) {
+        const uint32_t len = min((uint32_t)RC, maxR - c0);
+        __syncthreads();
+        for (uint32_t j = tid; j < len * npairs; j += T) {
+          const uint32_t b = j / len, jj = j - b * len;
+          double bv = 0.0, pv = 1.0;
+          if (S.status[b] == KST_OK && c0 + jj < S.ntrace[b]) {
+            const mgcd_pair_t t = args.trace[args.pairs[Gd.first + b].trace_off + c0 + jj];
+            bv = (double)t.value;
+            pv = (double)t.modulus;
+          }
+          S.tile[2 * (b * RC + jj)] = bv;
+          S.tile[2 * (b * RC + jj) + 1] = pv;
+        }
+        __syncthreads();
+        if (ew > 0) {
+          for (uint32_t j = 0; j < len; ++j) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+              if (e < ew && c0 + j < eR[e]) {
+                const double bv = S.tile[2 * (eb[e] * RC + j)], pv = S.tile[2 * (eb[e] * RC + j) + 1];
+                const double c2 = ra[e] - A::mulmod(bv, rc[e], RM[e]);  // |c2| <= m + 2^14
+                ra[e] = A::mulmod(rc[e], pv, RM[e]);
+                rs[e] = A::mulmod(rs[e], pv, RM[e]);
+                rc[e] = c2;
+              }
             }
           }
         }
       }
-    }
-    // u = a/s on the survivors, kept as (a, s) in place of (u, v).
+      // u = a/s on the survivors, kept as (a, s) in place of (u, v).
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      if ((uint32_t)k * T + tid < nsurv) {
-        args.res[ro[k]] = (uint32_t)canon(ra[k], RM[k]);
-        args.res[ro[k] + nq] = (uint32_t)canon(rs[k], RM[k]);
+      for (int e = 0; e < E; ++e) {
+        if (e0 + (uint32_t)e * T + tid < nsurv) {
+          args.res[ro[e]] = (uint32_t)canon(ra[e], RM[e]);
+          args.res[ro[e] + nq] = (uint32_t)canon(rs[e], RM[e]);
+        }
       }
     }
     __syncthreads();
@@ -914,6 +975,7 @@ __global__ __launch_bounds__(T) void gcd_kernel(KernelArgs args) {
     C.G = Gd.G;
     C.g = W.part;
     C.rnd = 0;
+    C.relay = Gd.G >= args.relay_min_g;
     C.bar = args.bar + Gd.bar_off;
     process_group<T, K, B, A>(args, Gd, S, C);
   }

@@ -71,7 +71,8 @@ struct DevBuf {
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
-    size_t want = std::max<size_t>(bytes, 256);
+    // Headroom: a reallocation (hipFree waits for the device) stays rare.
+    size_t want = std::max<size_t>(bytes + bytes / 4, 256);
     hipError_t e = hipMalloc(&p, want);
     if (e == hipSuccess) cap = want;
     return e;
@@ -83,6 +84,31 @@ struct DevBuf {
   }
   template <class T>
   T* as() const { return static_cast<T*>(p); }
+};
+
+// Grow-only pinned host buffer: copies to and from it are asynchronous.
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(bytes + bytes / 2, 4096);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
 };
 
 size_t trim(const uint32_t* x, size_t n) {
@@ -190,6 +216,7 @@ struct mgcd_ctx {
   bool autotune = false;  // MGCD_OPT_AUTOTUNE
   bool batch_graph = false;  // MGCD_OPT_BATCH_GRAPH
   bool tune_loaded = false;
+  uint32_t relay_min_g = 48;  // MGCD_OPT_RELAY_PARTS
   std::map<std::string, int> tuned;  // size class -> kernel variant
 };
 
@@ -248,7 +275,7 @@ int plan_items(mgcd_ctx* ctx, int vi, std::vector<PairDesc>& pd, std::vector<uin
       if (gd.G > maxres) return MGCD_ERR_TOO_MANY_THREADS;
       maxG = std::max(maxG, gd.G);
       gd.bar_off = bar;
-      if (gd.G > 1) bar += 2 * (size_t)gd.G * kv.B;
+      if (gd.G > 1) bar += 2 * ((size_t)gd.G + 1) * kv.B;
       groups.push_back(gd);
     }
     groups.back().count++;
@@ -520,6 +547,10 @@ int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value) {
     case MGCD_OPT_BATCH_GRAPH:
       ctx->batch_graph = value != 0;
       return MGCD_OK;
+    case MGCD_OPT_RELAY_PARTS:
+      if (value < 2) return MGCD_ERR_ARG;
+      ctx->relay_min_g = (uint32_t)std::min<int64_t>(value, 1 << 30);
+      return MGCD_OK;
     default:
       return MGCD_ERR_ARG;
   }
@@ -594,6 +625,7 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
   ka.res = ctx->res.as<uint32_t>();
   ka.out = ctx->out.as<PairOut>();
   ka.spin_limit = kSpinLimit;
+  ka.relay_min_g = ctx->relay_min_g;
   rc = launch(ctx, ls, ka);
   if (rc != MGCD_OK) return rc;
   PairOut po;
@@ -694,11 +726,11 @@ int class_of(uint32_t nq) {
 }
 
 // Batch tilings the tuner tries (threads, slots, pairs per group).
-constexpr int kTuneCandidates[][3] = {{256, 2, 16}, {512, 2, 16}, {256, 1, 16}, {512, 1, 16},
+constexpr int kTuneCandidates[][3] = {{512, 2, 16}, {256, 2, 16}, {256, 1, 16}, {512, 1, 16},
                                      {256, 2, 8},  {256, 1, 8},  {256, 1, 32}, {128, 1, 16}};
 
 int default_variant(const mgcd_ctx* ctx, int cls) {
-  return cls == 0 ? find_kernel_variant(256, 1, 16, ctx->wide) : find_kernel_variant(256, 2, 16, ctx->wide);
+  return cls == 0 ? find_kernel_variant(256, 1, 16, ctx->wide) : find_kernel_variant(512, 2, 16, ctx->wide);
 }
 
 }  // namespace
@@ -758,6 +790,7 @@ KernelArgs batch_args(const mgcd_batch* b, const BatchLaunch& l) {
   ka.res = b->res.as<uint32_t>();
   ka.out = b->out.as<PairOut>();
   ka.spin_limit = kSpinLimit;
+  ka.relay_min_g = ctx->relay_min_g;
   ka.nqueues = l.ls.nqueues;
   std::memcpy(ka.qoff, l.ls.qoff, sizeof ka.qoff);
   return ka;
@@ -782,16 +815,29 @@ int enqueue_launch(mgcd_batch* b, const BatchLaunch& l, bool cooperative) {
 // explicit variant.  b->pd is reordered launch by launch.
 int plan_batch(mgcd_batch* b, int forced_variant, const std::vector<int>& class_variant) {
   mgcd_ctx* ctx = b->ctx;
-  std::vector<std::vector<uint32_t>> members(forced_variant >= 0 ? 1 : kNumClasses);
-  for (uint32_t j = 0; j < b->pd.size(); ++j) members[forced_variant >= 0 ? 0 : class_of(b->pd[j].nq)].push_back(j);
+  std::vector<std::vector<uint32_t>> byclass(forced_variant >= 0 ? 1 : kNumClasses);
+  for (uint32_t j = 0; j < b->pd.size(); ++j) byclass[forced_variant >= 0 ? 0 : class_of(b->pd[j].nq)].push_back(j);
+  // Adjacent classes with the same tiling share one launch: its work queue
+  // then balances them against each other.
+  std::vector<std::vector<uint32_t>> members;
+  std::vector<int> set_variant;
+  for (size_t c = 0; c < byclass.size(); ++c) {
+    if (byclass[c].empty()) continue;
+    const int v = forced_variant >= 0 ? forced_variant : class_variant[c];
+    if (v < 0) return MGCD_ERR_ARG;
+    if (!set_variant.empty() && set_variant.back() == v) {
+      members.back().insert(members.back().end(), byclass[c].begin(), byclass[c].end());
+    } else {
+      members.push_back(byclass[c]);
+      set_variant.push_back(v);
+    }
+  }
   std::vector<PairDesc> all;
   std::vector<int64_t> pos(b->pd.size(), -1);
   size_t dig = 0, tr = 0, rs = 0, bar = 0;
   for (size_t c = 0; c < members.size(); ++c) {
-    if (members[c].empty()) continue;
     BatchLaunch l;
-    l.variant = forced_variant >= 0 ? forced_variant : class_variant[c];
-    if (l.variant < 0) return MGCD_ERR_ARG;
+    l.variant = set_variant[c];
     std::vector<PairDesc> cpd;
     for (uint32_t j : members[c]) cpd.push_back(b->pd[j]);
     std::vector<uint32_t> order;
@@ -1216,7 +1262,8 @@ int mgcd_autotune(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, c
       for (size_t i = 0; i < npairs; ++i) ok = ok && b->status[i] == MGCD_OK;
     }
     mgcd_batch_destroy(b);
-    if (ok && (best < 0 || t < best_t)) {
+    // Earlier candidates are kept unless a later one is clearly (3 %) faster.
+    if (ok && (best < 0 || t < 0.97 * best_t)) {
       best_t = t;
       best = vi;
     }
@@ -1291,15 +1338,23 @@ struct ServerSlot {
   hipStream_t stream = nullptr;
   hipEvent_t done_ev = nullptr;
   DevBuf limbs, pairs, groups, items, queue, bar, digits, trace, res, out;
+  // Pinned staging, so that neither direction of copy blocks the worker
+  // while other slots' kernels run.
+  PinnedBuf h_in, h_out, h_digits;
   std::vector<ServerReq*> reqs;
   std::vector<PairDesc> pd;
   std::vector<uint32_t> order;
-  std::vector<PairOut> h_out;
-  std::vector<mgcd_pair_t> h_digits;
   LaunchSet ls;
   uint64_t bytes = 0;
   bool busy = false;
+  int cls = 0;  // size class it serves: 0 small, 1 large
+  int variant = -1;
+  std::chrono::steady_clock::time_point t_launch;
 };
+
+// Requests of more than this many moduli (above ~200 kbit) are served by
+// their own slot, so that small requests never wait behind them in a launch.
+constexpr uint32_t kServeLargeNq = 16384;
 
 // Device bytes a pair holds while resident: limbs, residues, pivots,
 // digits, descriptors.
@@ -1313,15 +1368,15 @@ uint64_t footprint(size_t nu, size_t nv, uint32_t nq) {
 struct mgcd_server {
   mgcd_ctx* ctx = nullptr;
   uint64_t budget = 0, used = 0, peak = 0;
-  int variant = -1;
-  uint32_t grid_cap = 0;
+  int variant = -1, variant_large = -1;
   std::vector<ServerSlot> slots;
   std::mutex mu;
   std::condition_variable cv;
-  std::deque<ServerReq*> pending;
+  std::deque<ServerReq*> pending[2];  // by size class
   std::unordered_map<uint64_t, ServerReq*> reqs;
   uint64_t next_ticket = 1;
   bool stop = false;
+  bool trace = false;  // $MGCD_SERVER_TRACE: one stderr line per launch and completion
   std::thread worker;
   std::vector<double> latencies;
   uint64_t completed = 0;
@@ -1351,10 +1406,14 @@ int server_launch(mgcd_server* s, ServerSlot& sl) {
   std::vector<GroupDesc> groups;
   std::vector<WorkItem> items;
   sl.ls = LaunchSet();
-  int rc = plan_items(ctx, s->variant, sl.pd, sl.order, groups, items, sl.ls);
+  // A few large requests alone: the one-pair tiling (lower latency per
+  // iteration); otherwise groups of 16 sharing their inverses.
+  const int vi = sl.cls == 1 && sl.reqs.size() < 4 && s->variant_large >= 0 ? s->variant_large : s->variant;
+  sl.variant = vi;
+  int rc = plan_items(ctx, vi, sl.pd, sl.order, groups, items, sl.ls);
   if (rc != MGCD_OK) return rc;
   // Every slot's grid fits the device at the same time as the others'.
-  sl.ls.grid = std::min(sl.ls.grid, s->grid_cap);
+  sl.ls.grid = std::min(sl.ls.grid, (uint32_t)(ctx->max_resident[vi] / (int)s->slots.size()));
   uint32_t maxG = 0;
   for (const GroupDesc& g : groups) maxG = std::max(maxG, g.G);
   if (maxG > sl.ls.grid) return MGCD_ERR_TOO_MANY_THREADS;
@@ -1364,6 +1423,11 @@ int server_launch(mgcd_server* s, ServerSlot& sl) {
     sl.ls.qoff[1] = (uint32_t)items.size();
   }
   const size_t np = sl.pd.size();
+  const size_t in_bytes = packed.size() * 4 + np * sizeof(PairDesc) + groups.size() * sizeof(GroupDesc) +
+                          items.size() * sizeof(WorkItem) + 64;
+  if (sl.h_in.ensure(in_bytes) != hipSuccess || sl.h_out.ensure(np * sizeof(PairOut)) != hipSuccess ||
+      sl.h_digits.ensure(sl.ls.ndigits * 8 + 8) != hipSuccess)
+    return MGCD_ERR_NOMEM;
   if (sl.limbs.ensure(packed.size() * 4 + 4) != hipSuccess || sl.pairs.ensure(np * sizeof(PairDesc)) != hipSuccess ||
       sl.groups.ensure(groups.size() * sizeof(GroupDesc)) != hipSuccess ||
       sl.items.ensure(items.size() * sizeof(WorkItem)) != hipSuccess ||
@@ -1372,10 +1436,18 @@ int server_launch(mgcd_server* s, ServerSlot& sl) {
       sl.res.ensure(sl.ls.nres * 4) != hipSuccess || sl.out.ensure(np * sizeof(PairOut)) != hipSuccess)
     return MGCD_ERR_NOMEM;
   hipStream_t st = sl.stream;
-  HIP_TRY(hipMemcpyAsync(sl.limbs.p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(sl.pairs.p, sl.pd.data(), np * sizeof(PairDesc), hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(sl.groups.p, groups.data(), groups.size() * sizeof(GroupDesc), hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(sl.items.p, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st));
+  char* hin = sl.h_in.as<char>();
+  auto stage = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+    if (bytes == 0) return hipSuccess;
+    std::memcpy(hin, src, bytes);
+    const hipError_t e = hipMemcpyAsync(dst, hin, bytes, hipMemcpyHostToDevice, st);
+    hin += (bytes + 15) & ~(size_t)15;
+    return e;
+  };
+  HIP_TRY(stage(sl.limbs.p, packed.data(), packed.size() * 4));
+  HIP_TRY(stage(sl.pairs.p, sl.pd.data(), np * sizeof(PairDesc)));
+  HIP_TRY(stage(sl.groups.p, groups.data(), groups.size() * sizeof(GroupDesc)));
+  HIP_TRY(stage(sl.items.p, items.data(), items.size() * sizeof(WorkItem)));
   KernelArgs ka;
   std::memset(&ka, 0, sizeof ka);
   ka.limbs = sl.limbs.as<uint32_t>();
@@ -1393,19 +1465,18 @@ int server_launch(mgcd_server* s, ServerSlot& sl) {
   ka.res = sl.res.as<uint32_t>();
   ka.out = sl.out.as<PairOut>();
   ka.spin_limit = kSpinLimit;
+  ka.relay_min_g = ctx->relay_min_g;
   ka.nqueues = sl.ls.nqueues;
   std::memcpy(ka.qoff, sl.ls.qoff, sizeof ka.qoff);
   HIP_TRY(launch_fill_u32(ka.queue, kMaxQueues * kQueueStride, 0u, st));
   if (sl.ls.nbar) HIP_TRY(launch_fill_u64(ka.bar, sl.ls.nbar, 0ull, st));
   // Plain launch: the slots' grids together fit the device, so all of their
   // workgroups are resident at once while slots overlap.
-  const KernelVariant& kv = kernel_variant(s->variant);
+  const KernelVariant& kv = kernel_variant(vi);
   void* params[] = {&ka};
   HIP_TRY(launch_resident(kv.fn, sl.ls.grid, kv.T, params, st, false));
-  sl.h_out.resize(np);
-  sl.h_digits.resize(sl.ls.ndigits);
-  HIP_TRY(hipMemcpyAsync(sl.h_out.data(), sl.out.p, np * sizeof(PairOut), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(sl.h_digits.data(), sl.digits.p, sl.ls.ndigits * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(sl.h_out.p, sl.out.p, np * sizeof(PairOut), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(sl.h_digits.p, sl.digits.p, sl.ls.ndigits * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipEventRecord(sl.done_ev, st));
   return MGCD_OK;
 }
@@ -1420,9 +1491,9 @@ void server_finish(mgcd_server* s, ServerSlot& sl, int launch_rc) {
     const PairDesc& p = sl.pd[j];
     int st = launch_rc;
     if (st == MGCD_OK) {
-      const PairOut& o = sl.h_out[j];
+      const PairOut& o = sl.h_out.as<PairOut>()[j];
       st = o.status != KST_OK ? kst_to_status(o.status) : (o.ndig > p.nq ? MGCD_ERR_KERNEL : MGCD_OK);
-      if (st == MGCD_OK) digits_to_limbs(sl.h_digits.data() + p.dig_off, o.ndig, r->result);
+      if (st == MGCD_OK) digits_to_limbs(sl.h_digits.as<mgcd_pair_t>() + p.dig_off, o.ndig, r->result);
     }
     if (st == MGCD_ERR_REDUX || st == MGCD_ERR_RECOVERY || st == MGCD_ERR_TIMEOUT) {
       std::vector<uint32_t> g(r->v.size() + 1);
@@ -1446,6 +1517,9 @@ void server_loop(mgcd_server* s) {
       server_finish(s, sl, MGCD_OK);
       lk.lock();
       const auto now = std::chrono::steady_clock::now();
+      if (s->trace)
+        std::fprintf(stderr, "mgcd_server: slot %d done after %.3f ms\n", (int)(&sl - s->slots.data()),
+                     std::chrono::duration<double>(now - sl.t_launch).count() * 1e3);
       for (ServerReq* r : sl.reqs) {
         r->latency = std::chrono::duration<double>(now - r->t_submit).count();
         r->done = true;
@@ -1460,14 +1534,14 @@ void server_loop(mgcd_server* s) {
       progress = true;
       s->cv.notify_all();
     }
-    // Admission: waiting requests in arrival order, while they fit the
-    // budget, into a free slot.
+    // Admission: waiting requests of the slot's size class in arrival
+    // order, while they fit the budget, into a free slot.
     for (ServerSlot& sl : s->slots) {
-      if (sl.busy || s->pending.empty()) continue;
-      while (!s->pending.empty() && s->used + sl.bytes + s->pending.front()->bytes <= s->budget &&
-             sl.reqs.size() < 8192) {
-        ServerReq* r = s->pending.front();
-        s->pending.pop_front();
+      std::deque<ServerReq*>& q = s->pending[sl.cls];
+      if (sl.busy || q.empty()) continue;
+      while (!q.empty() && s->used + sl.bytes + q.front()->bytes <= s->budget && sl.reqs.size() < 8192) {
+        ServerReq* r = q.front();
+        q.pop_front();
         sl.reqs.push_back(r);
         sl.bytes += r->bytes;
       }
@@ -1475,9 +1549,13 @@ void server_loop(mgcd_server* s) {
       s->used += sl.bytes;
       s->peak = std::max(s->peak, s->used);
       sl.busy = true;
+      sl.t_launch = std::chrono::steady_clock::now();
       lk.unlock();
       const int rc = server_launch(s, sl);
       lk.lock();
+      if (s->trace)
+        std::fprintf(stderr, "mgcd_server: slot %d (class %d) launch of %zu requests, variant %d, rc %d\n",
+                     (int)(&sl - s->slots.data()), sl.cls, sl.reqs.size(), sl.variant, rc);
       if (rc != MGCD_OK) {  // nothing is in flight on this slot: answer now
         (void)hipStreamSynchronize(sl.stream);
         lk.unlock();
@@ -1499,12 +1577,13 @@ void server_loop(mgcd_server* s) {
     }
     bool busy = false;
     for (const ServerSlot& sl : s->slots) busy |= sl.busy;
-    if (s->stop && !busy && s->pending.empty()) break;
+    const bool waiting = !s->pending[0].empty() || !s->pending[1].empty();
+    if (s->stop && !busy && !waiting) break;
     if (!progress) {
       if (busy)
         s->cv.wait_for(lk, std::chrono::microseconds(100));
       else
-        s->cv.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+        s->cv.wait(lk, [&] { return s->stop || !s->pending[0].empty() || !s->pending[1].empty(); });
     }
   }
 }
@@ -1525,8 +1604,12 @@ int mgcd_server_create(mgcd_ctx* ctx, uint64_t budget_bytes, int slots, mgcd_ser
     delete s;
     return MGCD_ERR_TOO_MANY_THREADS;
   }
-  s->grid_cap = (uint32_t)(ctx->max_resident[s->variant] / slots);
+  s->variant_large = find_kernel_variant(256, 8, 1, ctx->wide);
+  const char* tr = std::getenv("MGCD_SERVER_TRACE");
+  s->trace = tr && *tr && *tr != '0';
   s->slots.resize(slots);
+  // With two or more slots the last one serves the large requests.
+  if (slots > 1) s->slots.back().cls = 1;
   for (ServerSlot& sl : s->slots) {
     if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&sl.done_ev, hipEventDisableTiming) != hipSuccess) {
@@ -1582,7 +1665,7 @@ int mgcd_server_submit(mgcd_server* s, const uint32_t* u, size_t nu, const uint3
     s->completed++;
     s->latencies.push_back(0.0);
   } else {
-    s->pending.push_back(r);
+    s->pending[s->slots.size() > 1 && r->nq > kServeLargeNq ? 1 : 0].push_back(r);
   }
   s->cv.notify_all();
   return MGCD_OK;
@@ -1642,6 +1725,7 @@ int mgcd_server_destroy(mgcd_server* s) {
     for (DevBuf* d : {&sl.limbs, &sl.pairs, &sl.groups, &sl.items, &sl.queue, &sl.bar, &sl.digits, &sl.trace, &sl.res,
                       &sl.out})
       d->release();
+    for (PinnedBuf* h : {&sl.h_in, &sl.h_out, &sl.h_digits}) h->release();
     if (sl.done_ev) (void)hipEventDestroy(sl.done_ev);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }

@@ -47,6 +47,7 @@ OPT_SINGLE_WORKGROUP = 3
 OPT_SINGLE_SLOTS = 4
 OPT_AUTOTUNE = 5
 OPT_BATCH_GRAPH = 6
+OPT_RELAY_PARTS = 7
 
 
 class Pair(ctypes.Structure):

@@ -15,13 +15,17 @@ def _mix(wl, n, seed):
     return wl.mixed(n=n, seed=seed, lo=1000, hi=200000)
 
 
-def test_size_classes_get_their_own_launches(ctx_for, mgcd, wl):
+def test_size_classes_get_their_own_tilings(ctx_for, mgcd, wl):
     ctx = ctx_for(32)
     pairs = _mix(wl, 96, 31)
     b = mgcd.Batch.from_pairs(ctx, pairs)
     b.run()
     launches = b.launches()
-    assert len(launches) >= 3
+    # Small pairs (N_Q <= 1024) run with their own tiling; adjacent classes
+    # with equal tilings share a launch.
+    assert len(launches) >= 2
+    assert all(x["tiling"] != y["tiling"] for x, y in zip(launches, launches[1:]))
+    assert launches[0]["nq"][1] <= 1024 < launches[1]["nq"][0]
     assert sum(x["pairs"] for x in launches) == sum(1 for u, v in pairs if u and v)
     for x in launches:
         assert x["device_s"] > 0

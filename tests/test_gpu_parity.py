@@ -65,6 +65,28 @@ def test_trace_multi_workgroup(ctx_for, oracle, mgcd):
     _check_trace(ctx_for(32), oracle, mgcd, u, v, 32)
 
 
+def test_trace_multi_workgroup_relay(mgcd, oracle, wl):
+    # The same kind of pair with the exchange relayed through part 0 (forced
+    # for every multi-part pair): pivots and digits still equal the oracle's.
+    ctx = mgcd.Context(32)
+    ctx.set_option(mgcd.OPT_RELAY_PARTS, 2)
+    try:
+        rnd = random.Random(12)
+        g = rnd.getrandbits(3000) | (1 << 2999)
+        u = g * (rnd.getrandbits(17000) | (1 << 16999))
+        v = g * (rnd.getrandbits(17000) | (1 << 16999))
+        if u < v:
+            u, v = v, u
+        _check_trace(ctx, oracle, mgcd, u, v, 32)
+        pairs = wl.headline(40, seed=13, nbits=20000, gbits=3000)
+        b = mgcd.Batch.from_pairs(ctx, pairs, workgroup=256, slots=2, group=16)
+        b.run()
+        assert b.results() == [math.gcd(x, y) for x, y in pairs]
+        b.close()
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("L", [32, 22])
 def test_small_and_ragged_sizes(ctx_for, mgcd, L):
     ctx = ctx_for(L)
