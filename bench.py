@@ -41,6 +41,12 @@ OPS_PER_REPLAY = 19             # per surviving modulus and recorded pivot: 3 mu
 OPS_PER_RECOVERY = 15           # 2 mulmod + reduce per modulus-digit
 
 
+# HBM bytes per 100k-bit pair (10k-bit shared factor, L = 32) of the batched kernel, measured with
+# rocprofv3 FETCH_SIZE / WRITE_SIZE on 512 such pairs and scaled by a copy kernel of known size
+# (FETCH_SIZE counts half the bytes, WRITE_SIZE all): 2 x 329,157 + 368,849 KiB per 512 pairs.
+HBM_BYTES_PER_HEADLINE_PAIR = (2 * 329157 + 368849) * 1024 / 512
+
+
 def elim_ops_per_unit(L, B):
     # one extended Euclid per modulus-iteration is shared by the B pairs of a group
     return OPS_PER_VALUE + EEA_STEPS[L] * OPS_PER_EEA_STEP / B
@@ -185,7 +191,10 @@ def main():
         "bit_exact": {"mpz_gcd_equal": exact_checked, "divisibility_checked": args.pairs},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_LANE_OPS / 1e12, 2),
                      "unit": "Tlane-op/s", "frac": round(achieved / (VALU_PEAK_LANE_OPS / 1e12), 4),
-                     "traffic": None, "kernel_ms": round(kern_s * 1e3, 3)},
+                     "traffic": (round(HBM_BYTES_PER_HEADLINE_PAIR * args.pairs)
+                                 if (args.bits, args.gbits, args.L) == (100000, 10000, 32) else None),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "kernel_ms": round(kern_s * 1e3, 3)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -300,12 +309,12 @@ def run_serve(args, m, w):
     n = 300
     out = []
     ctx = m.Context(32)
+    srv = m.Server(ctx, budget_bytes=64 << 30, slots=3)
     for ri, rate in enumerate(rates):
         sizes = w.serve_sizes(n, seed=5 + ri)
         reqs = [w.serve_pair(100 + ri, i, nb) for i, nb in enumerate(sizes)]
         rnd = random.Random(ri)
         gaps = [rnd.expovariate(rate) for _ in range(n)]
-        srv = m.Server(ctx, budget_bytes=64 << 30, slots=3)
         tickets = [None] * n
 
         def feed():
@@ -328,7 +337,6 @@ def run_serve(args, m, w):
             lat.append(l)
         span = time.perf_counter() - t0
         st = srv.stats()
-        srv.close()
 
         def pct(xs, f):
             xs = sorted(xs)
@@ -339,8 +347,9 @@ def run_serve(args, m, w):
                     for nb in sorted(set(sizes))}
         out.append({"offered_per_s": rate, "completed": n, "achieved_per_s": round(n / span, 2),
                     "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99), "by_bits": per_size,
-                    "peak_resident_mb": round(st["peak_bytes"] / 2**20, 1)})
+                    "peak_resident_mb_so_far": round(st["peak_bytes"] / 2**20, 1)})
         log(f"rate {rate}/s: {out[-1]}")
+    srv.close()
     ctx.close()
     print(json.dumps({
         "metric": "serving: p50/p99 latency and GCD/s at offered rates; bit-exact",

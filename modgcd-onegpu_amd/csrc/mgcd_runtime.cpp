@@ -1349,6 +1349,7 @@ struct ServerSlot {
   bool busy = false;
   int cls = 0;  // size class it serves: 0 small, 1 large
   int variant = -1;
+  bool own_stream = false;
   std::chrono::steady_clock::time_point t_launch;
 };
 
@@ -1610,9 +1611,17 @@ int mgcd_server_create(mgcd_ctx* ctx, uint64_t budget_bytes, int slots, mgcd_ser
   s->slots.resize(slots);
   // With two or more slots the last one serves the large requests.
   if (slots > 1) s->slots.back().cls = 1;
-  for (ServerSlot& sl : s->slots) {
-    if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.done_ev, hipEventDisableTiming) != hipSuccess) {
+  // Slot 0 runs on the context's own stream (the server owns the context):
+  // one hardware queue fewer, so the slots' streams do not end up sharing a
+  // queue, where a long kernel would hold back the others.
+  for (size_t i = 0; i < s->slots.size(); ++i) {
+    ServerSlot& sl = s->slots[i];
+    sl.own_stream = i > 0;
+    if (i == 0)
+      sl.stream = ctx->stream;
+    else if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess)
+      sl.stream = nullptr;
+    if (!sl.stream || hipEventCreateWithFlags(&sl.done_ev, hipEventDisableTiming) != hipSuccess) {
       mgcd_server_destroy(s);
       return MGCD_ERR_HIP;
     }
@@ -1727,7 +1736,7 @@ int mgcd_server_destroy(mgcd_server* s) {
       d->release();
     for (PinnedBuf* h : {&sl.h_in, &sl.h_out, &sl.h_digits}) h->release();
     if (sl.done_ev) (void)hipEventDestroy(sl.done_ev);
-    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.stream && sl.own_stream) (void)hipStreamDestroy(sl.stream);
   }
   for (auto& kv : s->reqs) delete kv.second;
   delete s;
