@@ -202,13 +202,18 @@ int mgcd_server_stats(mgcd_server* s, uint64_t* completed, double* p50_s, double
                       uint64_t* peak_bytes);
 int mgcd_server_destroy(mgcd_server* s);
 
-/* Exhaustive certification of the FP32-reciprocal quasi-quotient on this
- * device (the role of certifyQuoRemQuasi.cu):
+/* Exhaustive certification of the FP32-reciprocal quotient estimate that the
+ * modular inverse uses, on this device (the role of certifyQuoRemQuasi.cu).
+ * Postcondition, q = rint(fl32(x) * rcp(fl32(y))), r = x - q*y: |r| < y when
+ * x < 2^20 y, |r| < x otherwise (the inverse's Euclid then terminates).
  *   mode 0: every x in [1, 2^23), y in [1, 2^22)  (the float domain);
- *   mode 1: for width L, every y in [1, 2^L) and every quotient k, the two
- *           endpoints x = k*y and x = (k+1)*y - 1 (the estimate is monotone
- *           in x, so the endpoints decide every x in between).
- * *failures receives the number of (x, y) violating the postcondition. */
+ *   mode 1: for width L, every y in [1, 2^L) and every quotient class k, the
+ *           two endpoints x = k*y and x = (k+1)*y - 1 (the estimate is
+ *           monotone in x, so the endpoints decide every x in between);
+ *   mode 2: every float f in [1, 2^34): |rcp(f)*f - 1| <= 2^-23 and
+ *           rcp(-f) == -rcp(f) (the error bound and sign symmetry that the
+ *           inverse's termination and exactness arguments use).
+ * *failures receives the number of violations, *checked the number of cases. */
 int mgcd_certify_quasi_quotient(mgcd_ctx* ctx, int mode, int L, uint64_t* failures, uint64_t* checked);
 
 #ifdef __cplusplus

@@ -13,25 +13,30 @@
 //   reduce(x):    q = rint(x * minv), r = fma(-q, m, x)  (|x| < 2^52).
 //   canon(x):     x + m if x < 0 else x  -> [0, m).
 //
-// The only inverse computed is modinv (Knuth's Algorithm X with remainders
-// and cofactors kept exactly, modInv.h:141-188), whose quotients come from
-// the FP32 reciprocal v_rcp_f32 as quasi-quotients:
+// The only inverse computed is modinv, an extended Euclid with remainders
+// and cofactors kept exactly (the role of modInv.h:141-188), in its
+// least-absolute-remainder form: each step subtracts the multiple of y
+// nearest to x, so remainders are signed and |r| <= |y|/2 when the quotient
+// is exact.  That takes about ln(phi)/ln(2) = 0.69 of the steps of the
+// floor-quotient form (12.7 against 18.2 on average for m near 2^32;
+// Knuth, TAOCP vol. 2, 4.5.3, on least remainders).  Quotients come from the
+// FP32 reciprocal v_rcp_f32:
 //
-//   q~(x, y) = floor( fl32(x) * (rcp(fl32(y)) * QBIAS) ),  QBIAS = 1 - 2^-20
+//   q(x, y) = rint( fl32(x) * rcp(fl32(y)) )
 //
-// The bias makes q~ never exceed floor(x/y) (the combined relative error of
-// the conversions, v_rcp_f32 and two roundings is below 2^-21), so every
-// remainder x - q~ y is non-negative; it is computed exactly with an FMA,
-// and one exact integer correction (r >= y: r -= y) makes the quotient
-// exact whenever q~ >= floor(x/y) - 1.  The correction is what guarantees
-// progress: without it, x and y that round to the same float give q~ = 0
-// and the pair would only swap, forever.  When floor(x/y) >= 2^19 the
-// estimate may fall short by more; the remainder then stays >= y, the next
-// step has quotient 0 (a swap) and division resumes, still exact.
-// The "never exceeds" postcondition, and "short by at most one when
-// floor(x/y) < 2^19", are certified exhaustively on the device by
-// mgcd_certify_quasi_quotient (float domain x < 2^23, y < 2^22 point by
-// point, and all widths up to 32 bits through the monotone endpoints).
+// Any integer q keeps r == s*y (mod m) exact, so the estimate only decides
+// how fast |r| falls.  With rcp's relative error at most 2^-23 and three
+// roundings of 2^-24, |q - x/y| <= 1/2 + |x/y| * 2^-21.3, hence
+//   |r| < |y| when |x/y| < 2^20,  and  |r| < |x| otherwise,
+// and with |x| < |y| the step is a swap (q = 0) or leaves |r| <= |x|; so
+// |x| + |y| falls at least every second step and the loop ends.  These
+// postconditions of the actual device estimate are certified exhaustively
+// by mgcd_certify_quasi_quotient (modes 0 and 1), and the rcp error bound
+// and its sign symmetry over every float in [1, 2^34) by mode 2.
+// Below 2^22 (Arith22) x and y are exact floats and the estimate is off by
+// less than 1/4 (exact when y is a power of two), so |r| < 3|y|/4; with
+// |r_i s_(i+1) - r_(i+1) s_i| = m the cofactors then stay below 1.6 m, and
+// every FP32 FMA of the loop is exact.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,8 +44,6 @@
 
 namespace mgcd {
 namespace dev {
-
-constexpr float QBIAS = 1.0f - 0x1p-20f;
 
 struct ModD {
   double m;     // modulus
@@ -62,10 +65,9 @@ __device__ __forceinline__ int32_t to_signed(double r, ModD M) {
   return (int32_t)(x >= half ? x - M.m : x);
 }
 
-// Quasi-quotient from the FP32 reciprocal (see file comment).
-__device__ __forceinline__ float quasi_quo_f(float xf, float yf) {
-  const float c = __builtin_amdgcn_rcpf(yf) * QBIAS;
-  return __builtin_floorf(xf * c);
+// Nearest quotient estimate from the FP32 reciprocal (see file comment).
+__device__ __forceinline__ float near_quo_f(float xf, float yf) {
+  return __builtin_rintf(xf * __builtin_amdgcn_rcpf(yf));
 }
 
 // Moduli up to 32 bits: products need the two-product low part.
@@ -76,28 +78,29 @@ struct Arith32 {
     const double q = __builtin_rint(h * M.minv);
     return __builtin_fma(-q, M.m, h) + l;
   }
-  // x <- x mod y (exactly, see quasi_quo_f), sx <- sx - q sy, fx <- fl32(x).
+  // x <- x - q y, sx <- sx - q sy with q the nearest quotient estimate
+  // (exact: both are integers below 2^34), fx <- fl32(x).
   static __device__ __forceinline__ void eea_step(double& x, float& fx, double& sx, double y, float fy, double sy) {
-    double q = (double)quasi_quo_f(fx, fy);
-    const double r = __builtin_fma(-q, y, x);
-    q += (r >= y) ? 1.0 : 0.0;  // exact integer correction: q~ was floor(x/y) - 1
+    const double q = (double)near_quo_f(fx, fy);
     x = __builtin_fma(-q, y, x);
     sx = __builtin_fma(-q, sy, sx);
     fx = (float)x;
   }
   // y in [1, m) canonical; returns y^-1 mod m in [0, m).
   // Invariant r_i == s_i * y (mod m); unrolled by two so the pair swaps
-  // roles instead of moving registers; stops when a remainder is <= 1.
+  // roles instead of moving registers; stops when |r| <= 1.  The last
+  // nonzero remainder is +-1 (gcd(m, y) = 1): the inverse is its cofactor
+  // times its sign.
   static __device__ double modinv(double y, ModD M) {
     double r0 = M.m, r1 = y, s0 = 0.0, s1 = 1.0, s;
     float f0 = (float)M.m, f1 = (float)y;
     for (;;) {
       eea_step(r0, f0, s0, r1, f1, s1);
-      if (!(r0 > 1.0)) { s = (r0 == 1.0) ? s0 : s1; break; }
+      if (!(__builtin_fabs(r0) > 1.0)) { s = (r0 != 0.0) ? r0 * s0 : r1 * s1; break; }
       eea_step(r1, f1, s1, r0, f0, s0);
-      if (!(r1 > 1.0)) { s = (r1 == 1.0) ? s1 : s0; break; }
+      if (!(__builtin_fabs(r1) > 1.0)) { s = (r1 != 0.0) ? r1 * s1 : r0 * s0; break; }
     }
-    return s < 0.0 ? s + M.m : s;
+    return canon(reduce(s, M), M);
   }
 };
 
@@ -110,22 +113,19 @@ struct Arith22 {
     return __builtin_fma(-q, M.m, h);
   }
   static __device__ __forceinline__ void eea_step(float& x, float& sx, float y, float sy) {
-    float q = quasi_quo_f(x, y);
-    const float r = __builtin_fmaf(-q, y, x);
-    q += (r >= y) ? 1.0f : 0.0f;  // exact integer correction (q < 2^22: the sum is exact)
+    const float q = near_quo_f(x, y);
     x = __builtin_fmaf(-q, y, x);
     sx = __builtin_fmaf(-q, sy, sx);
   }
   static __device__ double modinv(double y, ModD M) {
-    const float m = (float)M.m;
-    float r0 = m, r1 = (float)y, s0 = 0.0f, s1 = 1.0f, s;
+    float r0 = (float)M.m, r1 = (float)y, s0 = 0.0f, s1 = 1.0f, s;
     for (;;) {
       eea_step(r0, s0, r1, s1);
-      if (!(r0 > 1.0f)) { s = (r0 == 1.0f) ? s0 : s1; break; }
+      if (!(__builtin_fabsf(r0) > 1.0f)) { s = (r0 != 0.0f) ? r0 * s0 : r1 * s1; break; }
       eea_step(r1, s1, r0, s0);
-      if (!(r1 > 1.0f)) { s = (r1 == 1.0f) ? s1 : s0; break; }
+      if (!(__builtin_fabsf(r1) > 1.0f)) { s = (r1 != 0.0f) ? r1 * s1 : r0 * s0; break; }
     }
-    return (double)(s < 0.0f ? s + m : s);
+    return canon(reduce((double)s, M), M);
   }
 };
 

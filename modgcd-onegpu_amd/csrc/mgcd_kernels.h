@@ -98,6 +98,7 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 // Certifier launches (see mgcd_certify_quasi_quotient).
 hipError_t launch_certify_float(unsigned long long* fail, unsigned long long* checked, int grid, hipStream_t s);
+hipError_t launch_certify_rcp(unsigned long long* fail, unsigned long long* checked, int grid, hipStream_t s);
 hipError_t launch_certify_endpoints(int L, uint64_t y_lo, uint64_t y_hi, unsigned long long* fail,
                                     unsigned long long* checked, int grid, hipStream_t s);
 

@@ -991,34 +991,49 @@ __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
 }
 
 // ---------------------------------------------------------------------------
-// Certifier of quasi_quo_f (the role of certifyQuoRemQuasi.cu:25-48).
-// Postcondition for integers x >= 0, y >= 1 with Q = floor(x/y):
-//   q~ <= Q always, and q~ >= Q - 1 whenever Q < 2^19.
-// Inputs reach quasi_quo_f the way modinv passes them: fl32 of an exact
+// Certifier of the quotient estimate near_quo_f that the modular inverse
+// uses (the role of certifyQuoRemQuasi.cu:25-48).  Postcondition for
+// integers x >= 0, y >= 1 with q = near_quo_f(fl32(x), fl32(y)) and
+// r = x - q*y:
+//   |r| < y whenever x < 2^20 y,   |r| < x whenever x >= 2^20 y,
+// which is what makes |x| + |y| of the inverse's Euclid fall; the sign
+// symmetry of the estimate (mode 2) extends it to negative x and y.
+// Inputs reach near_quo_f the way modinv passes them: fl32 of an exact
 // integer (round to nearest even).
-__device__ __forceinline__ bool qq_bad(uint64_t x, uint64_t y) {
-  const float q = quasi_quo_f((float)(double)x, (float)(double)y);
+__device__ __forceinline__ bool nq_bad(uint64_t x, uint64_t y) {
+  const float q = near_quo_f((float)(double)x, (float)(double)y);
   const int64_t r = (int64_t)x - (int64_t)q * (int64_t)y;
-  if (r < 0) return true;
-  return r >= (int64_t)(2 * y) && x < (y << 19);
+  const uint64_t ar = (uint64_t)(r < 0 ? -r : r);
+  return x < (y << 20) ? ar >= y : ar >= x;
+}
+
+// Quotient class k of y, x in [k*y, xh] with xh < (k+1)*y: q is
+// non-decreasing in x (conversion, multiplication by a positive constant
+// and rounding are monotone), so the estimates at the two ends bound every
+// x of the class.  k < 2^20: q(k*y) = k and q(xh) in {k, k+1} give |r| < y
+// throughout; k >= 2^20: 1 <= q <= 2k - 1 gives 0 < q*y < 2x, i.e. |r| < x.
+__device__ __forceinline__ bool class_bad(uint64_t k, uint64_t y, uint64_t xh) {
+  const int64_t qa = (int64_t)near_quo_f((float)(double)(k * y), (float)(double)y);
+  const int64_t qb = (int64_t)near_quo_f((float)(double)xh, (float)(double)y);
+  const int64_t ki = (int64_t)k;
+  if (k < (1u << 20)) return qa != ki || qb < ki || qb > ki + 1;
+  return qa < 1 || qb > 2 * ki - 1;
 }
 
 // Mode 0: every x in [1, 2^23), y in [1, 2^22), point by point.
 __global__ void certify_float_kernel(unsigned long long* fail, unsigned long long* checked) {
   unsigned long long bad = 0, cnt = 0;
   for (uint32_t y = blockIdx.x * blockDim.x + threadIdx.x + 1; y < (1u << 22); y += gridDim.x * blockDim.x) {
-    for (uint32_t x = 1; x < (1u << 23); ++x) bad += qq_bad(x, y);
+    for (uint32_t x = 1; x < (1u << 23); ++x) bad += nq_bad(x, y);
     cnt += (1u << 23) - 1;
   }
   if (bad) atomicAdd(fail, bad);
   atomicAdd(checked, cnt);
 }
 
-// Mode 1: width L, y in [y_lo, y_hi), every quotient k of x in [0, 2^L):
-// endpoints x = k*y and x = min((k+1)*y - 1, 2^L - 1).  q~(x, y) is
-// non-decreasing in x (conversion, multiplication by a positive constant
-// and rounding are monotone), so the endpoints bound every x of class k.
-// A launch covers one octave of y; unit u is (y = y_lo + (u >> cshift),
+// Mode 1: width L, y in [y_lo, y_hi), every quotient class k of x in
+// [0, 2^L) through its endpoints x = k*y and x = min((k+1)*y - 1, 2^L - 1)
+// (class_bad).  A launch covers one octave of y; unit u is (y = y_lo + (u >> cshift),
 // chunk = u & (2^cshift - 1)) of 4096 quotients; chunks past kmax(y) skip.
 __global__ void certify_endpoints_kernel(int L, uint64_t y_lo, uint64_t y_hi, int cshift, unsigned long long* fail,
                                          unsigned long long* checked) {
@@ -1035,14 +1050,31 @@ __global__ void certify_endpoints_kernel(int L, uint64_t y_lo, uint64_t y_hi, in
     if (k0 > kmax) continue;
     const uint64_t k1 = min(kmax, k0 + CH - 1);
     for (uint64_t k = k0; k <= k1; ++k) {
-      if (k > 0) bad += qq_bad(k * y, y);
-      const uint64_t xh = min((k + 1) * y - 1, xmax);
-      if (xh > 0) bad += qq_bad(xh, y);
+      bad += class_bad(k, y, min((k + 1) * y - 1, xmax));
       cnt += 2;
     }
   }
   if (bad) atomicAdd(fail, bad);
   if (cnt) atomicAdd(checked, cnt);
+}
+
+// Mode 2: every float f in [1, 2^34) (every |y| the inverse divides by):
+// |rcp(f) * f - 1| <= 2^-23, evaluated exactly as one FP64 FMA (two 24-bit
+// significands), and rcp(-f) == -rcp(f).
+__global__ void certify_rcp_kernel(unsigned long long* fail, unsigned long long* checked) {
+  constexpr uint32_t kLo = 0x3f800000u;  // 1.0f
+  constexpr uint32_t kHi = 0x50800000u;  // 2^34
+  unsigned long long bad = 0, cnt = 0;
+  for (uint32_t b = kLo + blockIdx.x * blockDim.x + threadIdx.x; b < kHi; b += gridDim.x * blockDim.x) {
+    const float f = __uint_as_float(b);
+    const float r = __builtin_amdgcn_rcpf(f);
+    const float rn = __builtin_amdgcn_rcpf(-f);
+    const double e = __builtin_fma((double)r, (double)f, -1.0);
+    bad += (__builtin_fabs(e) > 0x1p-23) | (rn != -r);
+    ++cnt;
+  }
+  if (bad) atomicAdd(fail, bad);
+  atomicAdd(checked, cnt);
 }
 
 }  // namespace
@@ -1083,6 +1115,10 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
 
 hipError_t launch_certify_float(unsigned long long* fail, unsigned long long* checked, int grid, hipStream_t s) {
   hipLaunchKernelGGL(certify_float_kernel, dim3(grid), dim3(256), 0, s, fail, checked);
+  return hipGetLastError();
+}
+hipError_t launch_certify_rcp(unsigned long long* fail, unsigned long long* checked, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(certify_rcp_kernel, dim3(grid), dim3(256), 0, s, fail, checked);
   return hipGetLastError();
 }
 hipError_t launch_certify_endpoints(int L, uint64_t y_lo, uint64_t y_hi, unsigned long long* fail,

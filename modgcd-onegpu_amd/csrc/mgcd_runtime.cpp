@@ -66,12 +66,16 @@ thread_local std::string g_last_error;
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
-  hipError_t ensure(size_t bytes) {
+  // retired: where the old allocation goes instead of hipFree (which waits
+  // for the whole device), for callers that must not block while other
+  // streams' kernels run; the caller frees them later.
+  hipError_t ensure(size_t bytes, std::vector<void*>* retired = nullptr) {
     if (bytes <= cap) return hipSuccess;
-    if (p) (void)hipFree(p);
+    if (p && retired) retired->push_back(p);
+    else if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
-    // Headroom: a reallocation (hipFree waits for the device) stays rare.
+    // Headroom: a reallocation stays rare.
     size_t want = std::max<size_t>(bytes + bytes / 4, 256);
     hipError_t e = hipMalloc(&p, want);
     if (e == hipSuccess) cap = want;
@@ -90,9 +94,10 @@ struct DevBuf {
 struct PinnedBuf {
   void* p = nullptr;
   size_t cap = 0;
-  hipError_t ensure(size_t bytes) {
+  hipError_t ensure(size_t bytes, std::vector<void*>* retired = nullptr) {
     if (bytes <= cap) return hipSuccess;
-    if (p) (void)hipHostFree(p);
+    if (p && retired) retired->push_back(p);
+    else if (p) (void)hipHostFree(p);
     p = nullptr;
     cap = 0;
     const size_t want = std::max<size_t>(bytes + bytes / 2, 4096);
@@ -1279,7 +1284,7 @@ int mgcd_autotune(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, c
 
 // ---------------------------------------------------------------------------
 int mgcd_certify_quasi_quotient(mgcd_ctx* ctx, int mode, int L, uint64_t* failures, uint64_t* checked) {
-  if (!ctx || !failures || !checked || (mode != 0 && mode != 1) || (mode == 1 && (L < 2 || L > 32)))
+  if (!ctx || !failures || !checked || mode < 0 || mode > 2 || (mode == 1 && (L < 2 || L > 32)))
     return MGCD_ERR_ARG;
   HIP_TRY(hipSetDevice(ctx->device));
   DevBuf cnt;
@@ -1292,6 +1297,8 @@ int mgcd_certify_quasi_quotient(mgcd_ctx* ctx, int mode, int L, uint64_t* failur
     const int grid = ctx->num_cus * 16;
     if (mode == 0) {
       HIP_TRY(launch_certify_float(d, d + 1, grid, s));
+    } else if (mode == 2) {
+      HIP_TRY(launch_certify_rcp(d, d + 1, grid, s));
     } else {
       for (int b = 0; b < L; ++b) {
         const uint64_t lo = 1ull << b;           // octave [2^b, 2^(b+1)) of y
@@ -1341,6 +1348,10 @@ struct ServerSlot {
   // Pinned staging, so that neither direction of copy blocks the worker
   // while other slots' kernels run.
   PinnedBuf h_in, h_out, h_digits;
+  // Buffers outgrown while serving, freed with the server: hipFree and
+  // hipHostFree wait for every stream, which would stall this slot behind
+  // the other slots' kernels.
+  std::vector<void*> retired_dev, retired_host;
   std::vector<ServerReq*> reqs;
   std::vector<PairDesc> pd;
   std::vector<uint32_t> order;
@@ -1369,7 +1380,7 @@ uint64_t footprint(size_t nu, size_t nv, uint32_t nq) {
 struct mgcd_server {
   mgcd_ctx* ctx = nullptr;
   uint64_t budget = 0, used = 0, peak = 0;
-  int variant = -1, variant_large = -1;
+  int variant = -1;
   std::vector<ServerSlot> slots;
   std::mutex mu;
   std::condition_variable cv;
@@ -1407,9 +1418,24 @@ int server_launch(mgcd_server* s, ServerSlot& sl) {
   std::vector<GroupDesc> groups;
   std::vector<WorkItem> items;
   sl.ls = LaunchSet();
-  // A few large requests alone: the one-pair tiling (lower latency per
-  // iteration); otherwise groups of 16 sharing their inverses.
-  const int vi = sl.cls == 1 && sl.reqs.size() < 4 && s->variant_large >= 0 ? s->variant_large : s->variant;
+  // A few large requests alone: the one-pair tiling with the fewest moduli
+  // per thread whose workgroup count fits this slot's share of the device
+  // (lower latency per iteration); otherwise groups of 16 sharing their
+  // inverses.
+  int vi = s->variant;
+  if (sl.cls == 1 && sl.reqs.size() < 4) {
+    uint32_t nq_max = 0;
+    for (const ServerReq* r : sl.reqs) nq_max = std::max(nq_max, r->nq);
+    for (int K : {1, 2, 4, 8}) {
+      const int v = find_kernel_variant(256, K, 1, ctx->wide);
+      if (v < 0) continue;
+      const uint32_t G = (nq_max + 256 * K - 1) / (256 * K);
+      if (G <= (uint32_t)(ctx->max_resident[v] / (int)s->slots.size())) {
+        vi = v;
+        break;
+      }
+    }
+  }
   sl.variant = vi;
   int rc = plan_items(ctx, vi, sl.pd, sl.order, groups, items, sl.ls);
   if (rc != MGCD_OK) return rc;
@@ -1426,15 +1452,19 @@ int server_launch(mgcd_server* s, ServerSlot& sl) {
   const size_t np = sl.pd.size();
   const size_t in_bytes = packed.size() * 4 + np * sizeof(PairDesc) + groups.size() * sizeof(GroupDesc) +
                           items.size() * sizeof(WorkItem) + 64;
-  if (sl.h_in.ensure(in_bytes) != hipSuccess || sl.h_out.ensure(np * sizeof(PairOut)) != hipSuccess ||
-      sl.h_digits.ensure(sl.ls.ndigits * 8 + 8) != hipSuccess)
+  std::vector<void*>* rh = &sl.retired_host;
+  std::vector<void*>* rd = &sl.retired_dev;
+  if (sl.h_in.ensure(in_bytes, rh) != hipSuccess || sl.h_out.ensure(np * sizeof(PairOut), rh) != hipSuccess ||
+      sl.h_digits.ensure(sl.ls.ndigits * 8 + 8, rh) != hipSuccess)
     return MGCD_ERR_NOMEM;
-  if (sl.limbs.ensure(packed.size() * 4 + 4) != hipSuccess || sl.pairs.ensure(np * sizeof(PairDesc)) != hipSuccess ||
-      sl.groups.ensure(groups.size() * sizeof(GroupDesc)) != hipSuccess ||
-      sl.items.ensure(items.size() * sizeof(WorkItem)) != hipSuccess ||
-      sl.queue.ensure(kMaxQueues * kQueueStride * 4) != hipSuccess || sl.bar.ensure((sl.ls.nbar + 1) * 8) != hipSuccess ||
-      sl.digits.ensure(sl.ls.ndigits * 8) != hipSuccess || sl.trace.ensure(sl.ls.ntrace * 8) != hipSuccess ||
-      sl.res.ensure(sl.ls.nres * 4) != hipSuccess || sl.out.ensure(np * sizeof(PairOut)) != hipSuccess)
+  if (sl.limbs.ensure(packed.size() * 4 + 4, rd) != hipSuccess ||
+      sl.pairs.ensure(np * sizeof(PairDesc), rd) != hipSuccess ||
+      sl.groups.ensure(groups.size() * sizeof(GroupDesc), rd) != hipSuccess ||
+      sl.items.ensure(items.size() * sizeof(WorkItem), rd) != hipSuccess ||
+      sl.queue.ensure(kMaxQueues * kQueueStride * 4, rd) != hipSuccess ||
+      sl.bar.ensure((sl.ls.nbar + 1) * 8, rd) != hipSuccess || sl.digits.ensure(sl.ls.ndigits * 8, rd) != hipSuccess ||
+      sl.trace.ensure(sl.ls.ntrace * 8, rd) != hipSuccess || sl.res.ensure(sl.ls.nres * 4, rd) != hipSuccess ||
+      sl.out.ensure(np * sizeof(PairOut), rd) != hipSuccess)
     return MGCD_ERR_NOMEM;
   hipStream_t st = sl.stream;
   char* hin = sl.h_in.as<char>();
@@ -1605,7 +1635,6 @@ int mgcd_server_create(mgcd_ctx* ctx, uint64_t budget_bytes, int slots, mgcd_ser
     delete s;
     return MGCD_ERR_TOO_MANY_THREADS;
   }
-  s->variant_large = find_kernel_variant(256, 8, 1, ctx->wide);
   const char* tr = std::getenv("MGCD_SERVER_TRACE");
   s->trace = tr && *tr && *tr != '0';
   s->slots.resize(slots);
@@ -1735,6 +1764,8 @@ int mgcd_server_destroy(mgcd_server* s) {
                       &sl.out})
       d->release();
     for (PinnedBuf* h : {&sl.h_in, &sl.h_out, &sl.h_digits}) h->release();
+    for (void* p : sl.retired_dev) (void)hipFree(p);
+    for (void* p : sl.retired_host) (void)hipHostFree(p);
     if (sl.done_ev) (void)hipEventDestroy(sl.done_ev);
     if (sl.stream && sl.own_stream) (void)hipStreamDestroy(sl.stream);
   }
