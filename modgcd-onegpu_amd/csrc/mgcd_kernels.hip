@@ -214,7 +214,17 @@ __device__ __forceinline__ double class_min(double x) {
 template <int T, int K, int B>
 struct Shared {
   static constexpr int NW = T / 64;
-  double tile[TILE];          // limb tile / pivot tile (value, modulus)
+  // Staging for (a) and (c), and during (b) the group's moduli (when N_Q
+  // fits), so that a pivot's modulus is an LDS read instead of a global one
+  // on the path between two rounds.
+  static constexpr uint32_t MODS = (TILE * 8 + B * T * K * 2) / 4;
+  union {
+    struct {
+      double tile[TILE];         // limb tile / pivot tile (value, modulus)
+      uint16_t surv[B * T * K];  // survivors of the group: pair << 11 | local offset
+    } st;
+    uint32_t mods[MODS];         // (b): modulus of every index < N_Q
+  } u;
   double red[B][NW];          // per-wave minima
   double bb[B], pp[B];        // current pivot of each pair: value, modulus
   uint32_t pidx[B];           // its modulus index (NONE_IDX: none)
@@ -223,7 +233,6 @@ struct Shared {
   int32_t status[B];
   uint32_t ntrace[B];
   double sel[B];              // recovery: the owner's (index, value) key
-  uint16_t surv[B * T * K];   // survivors of the group: pair << 11 | local offset
   uint32_t retire[T * K];     // bit b: this part's modulus is pair b's pivot
   uint32_t item;
 };
@@ -349,23 +358,26 @@ __device__ void residues(const uint32_t* (&x)[NOP], const uint32_t (&n)[NOP], do
 struct PairState {
   int32_t live, status, remaining, ubits, vbits, pval;
   uint32_t niter, ndig;
+  uint32_t trace_cap;  // the pair's trace slots, kept here so that the
+  uint64_t trace_off;  // per-iteration path loads nothing from its descriptor
 };
 
 // Records pivot (idx, val) as iteration niter of the pair: trace entry
 // (leader workgroup), and the next round's pivot in LDS.
 template <int T, int K, int B>
-__device__ __forceinline__ void publish_pivot(const KernelArgs& args, const PairDesc& P, bool lead, uint32_t b,
+__device__ __forceinline__ void publish_pivot(const KernelArgs& args, bool lead, bool lds_mods, uint32_t b,
                                               PairState& ps, uint32_t idx, int32_t val, Shared<T, K, B>& S) {
-  if (lead && ps.niter < P.trace_cap) {
+  const uint32_t mu = lds_mods ? S.u.mods[idx] : args.mod_u32[idx];
+  if (lead && ps.niter < ps.trace_cap) {
     mgcd_pair_t e;
-    e.modulus = args.mod_u32[idx];
+    e.modulus = mu;
     e.value = val;
-    args.trace[P.trace_off + ps.niter] = e;
+    args.trace[ps.trace_off + ps.niter] = e;
   }
   ps.niter++;
   ps.pval = val;
   S.bb[b] = (double)val;
-  S.pp[b] = args.mod_d[idx];
+  S.pp[b] = (double)mu;
   S.pidx[b] = idx;
 }
 
@@ -420,7 +432,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       ro[2 * q + 1] = P.res_off + nq;
     }
     double r[2 * PP][K];
-    residues<T, K, 2 * PP>(x, nx, r, M, rows, S.tile);
+    residues<T, K, 2 * PP>(x, nx, r, M, rows, S.u.st.tile);
 #pragma unroll
     for (int o = 0; o < 2 * PP; ++o) {
       if (b0 + o / 2 >= npairs) break;
@@ -431,6 +443,11 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   }
 
   // ---- (b) elimination ----------------------------------------------------
+  // The moduli table of the group in LDS, over (a)'s staging tile.
+  const bool lds_mods = nq <= Shared<T, K, B>::MODS;
+  __syncthreads();
+  if (lds_mods)
+    for (uint32_t i = tid; i < nq; i += T) S.u.mods[i] = args.mod_u32[i];
   // Initial t = u/v (getGcdKernel.cu:369), one batched inverse per slot.
   // Keys are reduced over the wave slot by slot (reduce-scatter), so only
   // one slot's B keys are ever held: wkey in lane l is the wave's minimum
@@ -474,6 +491,8 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   ps.status = KST_NOTRUN;
   ps.remaining = ps.ubits = ps.vbits = ps.pval = 0;
   ps.niter = ps.ndig = 0;
+  ps.trace_cap = 0;
+  ps.trace_off = 0;
   if (wave == 0 && lane < npairs) {
     const PairDesc& Pl = args.pairs[Gd.first + lane];
     ps.live = 1;
@@ -481,6 +500,8 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
     ps.remaining = (int32_t)nq;                   // totalModuliRemaining (:350)
     ps.ubits = (int32_t)((Pl.nu + 1) * 32);       // :351
     ps.vbits = (int32_t)((Pl.nv + 1) * 32);       // :352
+    ps.trace_cap = Pl.trace_cap;
+    ps.trace_off = Pl.trace_off;
   }
 
   // Reduces the wave minima over the group; wave 0 lane b < B gets pair b's.
@@ -520,7 +541,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
           uint32_t idx;
           int32_t val;
           elim_decode(key_bits(kr), idx, val);
-          publish_pivot(args, args.pairs[Gd.first + lane], lead, lane, ps, idx, val, S);
+          publish_pivot(args, lead, lds_mods, lane, ps, idx, val, S);
           mark_retire(lane, idx);
         }
       }
@@ -610,7 +631,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
           if (val == INT32_MIN) {
             ps.live = 0;  // infinity: done (:402)
           } else {
-            publish_pivot(args, args.pairs[Gd.first + lane], lead, lane, ps, idx, val, S);
+            publish_pivot(args, lead, lds_mods, lane, ps, idx, val, S);
             mark_retire(lane, idx);
           }
         }
@@ -662,7 +683,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         for (int w = 0; w < NW; ++w) {
           const uint64_t word = S.actw[b][k][w];
           if ((uint32_t)w == wave && ((word >> lane) & 1))
-            S.surv[nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(word >> 32),
+            S.u.st.surv[nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(word >> 32),
                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)word, 0))] =
                 (uint16_t)((b << 11) | (uint32_t)(k * T + tid));
           nsurv += (uint32_t)__builtin_popcountll(word);
@@ -687,7 +708,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         eb[e] = 0;
         eR[e] = 0;
         if (j < nsurv) {
-          const uint32_t ent = S.surv[j];
+          const uint32_t ent = S.u.st.surv[j];
           const uint32_t idx = base + (ent & 0x7FFu);
           eb[e] = ent >> 11;
           eR[e] = S.ntrace[eb[e]];
@@ -714,8 +735,8 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
             bv = (double)t.value;
             pv = (double)t.modulus;
           }
-          S.tile[2 * (b * RC + jj)] = bv;
-          S.tile[2 * (b * RC + jj) + 1] = pv;
+          S.u.st.tile[2 * (b * RC + jj)] = bv;
+          S.u.st.tile[2 * (b * RC + jj) + 1] = pv;
         }
         __syncthreads();
         if (ew > 0) {
@@ -723,7 +744,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
 #pragma unroll
             for (int e = 0; e < E; ++e) {
               if (e < ew && c0 + j < eR[e]) {
-                const double bv = S.tile[2 * (eb[e] * RC + j)], pv = S.tile[2 * (eb[e] * RC + j) + 1];
+                const double bv = S.u.st.tile[2 * (eb[e] * RC + j)], pv = S.u.st.tile[2 * (eb[e] * RC + j) + 1];
                 const double c2 = ra[e] - A::mulmod(bv, rc[e], RM[e]);  // |c2| <= m + 2^14
                 ra[e] = A::mulmod(rc[e], pv, RM[e]);
                 rs[e] = A::mulmod(rs[e], pv, RM[e]);
