@@ -105,9 +105,21 @@ enum {
    * exchange their per-iteration minima through part 0 (two hops, traffic
    * linear in the parts) instead of every part reading every part's slot
    * (one hop, traffic quadratic in the parts). */
-  MGCD_OPT_RELAY_PARTS = 7
+  MGCD_OPT_RELAY_PARTS = 7,
+  /* 1 (default): the signed mixed-radix digits become positional limbs on
+   * the device, by a product tree (batches; one-pair calls above 64 digits),
+   * and only |G| is copied back.  0: the digits are copied back and
+   * converted by the host Horner loop (GmpCudaDevice-gcd.cu:113-129). */
+  MGCD_OPT_DEVICE_CONVERT = 8
 };
 int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value);
+/* Signed mixed-radix digits (least significant first, the layout the kernel
+ * writes: G = g_0 + m_0 (g_1 + m_1 (...))) -> |G| as limbs, converted on the
+ * device by the product tree that MGCD_OPT_DEVICE_CONVERT enables (the role
+ * of the host Horner loop, GmpCudaDevice-gcd.cu:113-129).  Each |g_i| must
+ * be at most (m_i + 1) / 2, as the kernel's digits are. */
+int mgcd_mixed_radix_to_limbs(mgcd_ctx* ctx, const mgcd_pair_t* digits, size_t k, uint32_t* g, size_t cap,
+                              size_t* ng);
 /* Blocks until all work queued on the context's device has finished. */
 int mgcd_ctx_synchronize(mgcd_ctx* ctx);
 /* Seconds the last gcd kernel launch of this context took (HIP events). */

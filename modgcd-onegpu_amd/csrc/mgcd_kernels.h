@@ -32,7 +32,7 @@ struct GroupDesc {
   uint32_t count;
   uint32_t nq;         // N_Q of every pair of the group
   uint32_t G;          // workgroups the group is split over
-  uint64_t bar_off;    // first barrier slot (2*(G+1)*B uint64 slots), if G > 1
+  uint64_t bar_off;    // first barrier slot (2*(G+1)*B exchange slots, then G residue-job words), if G > 1
 };
 
 // One unit of the persistent work queue: part `part` of group `group`.
@@ -95,6 +95,32 @@ int find_kernel_variant(int T, int K, int B, int wide);
 // Small helper kernels.
 hipError_t launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+
+// Mixed-radix digits -> positional limbs on the device (mgcd_convert.hip):
+// a product tree over each pair's digits, level j holding nodes of 2^j
+// digits in two alternating level buffers.
+struct MrPair {
+  uint64_t dig_off;     // the pair's digits (least significant first) in the digit buffer
+  uint64_t buf_off;     // its region (8k + 8 limbs) in each level buffer
+  uint64_t out_off;     // |G| limbs, k + 1 of room, in the result buffer
+  uint32_t k;           // digits
+  uint32_t root_level;  // ceil(log2 k): the level that holds V(0, k)
+};
+struct MrArgs {
+  const mgcd_pair_t* digits;
+  const MrPair* pairs;
+  uint32_t npairs;
+  uint32_t j;           // input level (0: the digits)
+  const uint32_t* cum;  // output nodes of pairs [0, p) at level j + 1; npairs + 1 entries
+  const uint32_t* in;   // level j (j >= 1)
+  uint32_t* out;        // level j + 1
+};
+// Levels below this (nodes of at most 16 digits) combine one node per thread.
+constexpr uint32_t kMrSmallLevels = 5;
+hipError_t launch_mr_level(const MrArgs& a, uint32_t ntasks, hipStream_t s);
+// |G| per pair from its root: magnitude limbs at out_off, length in lens[p].
+hipError_t launch_mr_emit(const MrArgs& a, const uint32_t* buf0, const uint32_t* buf1, uint32_t* res, uint32_t* lens,
+                          hipStream_t s);
 
 // Certifier launches (see mgcd_certify_quasi_quotient).
 hipError_t launch_certify_float(unsigned long long* fail, unsigned long long* checked, int grid, hipStream_t s);

@@ -240,7 +240,7 @@ struct Shared {
 struct Ctx {
   uint32_t G, g;             // parts of the group, this part
   uint32_t rnd;              // cross-workgroup round counter
-  unsigned long long* bar;   // this group's 2*(G+1)*B slots
+  unsigned long long* bar;   // this group's 2*(G+1)*B exchange slots, then G residue-job words
   uint32_t spin_limit;
   bool relay;                // exchange through part 0 (large G)
 };
@@ -398,6 +398,118 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   const bool lead = C.g == 0;
   const int L = args.L;
 
+  // ---- (a) residues of u and v, stored canonical for (c) -----------------
+  // The residues of part p (PP pairs, 2 PP operands, at a time).
+  auto residues_of_part = [&](uint32_t p) {
+    const uint32_t pb = (uint32_t)(((uint64_t)p * nq) / C.G);
+    const uint32_t pc = (uint32_t)(((uint64_t)(p + 1) * nq) / C.G) - pb;
+    ModD PM[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t j = (uint32_t)k * T + tid;
+      PM[k].m = j < pc ? args.mod_d[pb + j] : 3.0;
+      PM[k].minv = j < pc ? args.mod_inv[pb + j] : 1.0 / 3.0;
+    }
+    constexpr int PP = (B >= 2 && K <= 2) ? 2 : 1;
+    for (uint32_t b0 = 0; b0 < npairs; b0 += PP) {
+      const uint32_t* x[2 * PP];
+      uint32_t nx[2 * PP];
+      uint64_t ro[2 * PP];
+#pragma unroll
+      for (int q = 0; q < PP; ++q) {
+        const bool here = b0 + q < npairs;
+        const PairDesc& P = args.pairs[Gd.first + (here ? b0 + q : b0)];
+        x[2 * q] = args.limbs + P.u_off;
+        x[2 * q + 1] = args.limbs + P.v_off;
+        nx[2 * q] = here ? P.nu : 0u;
+        nx[2 * q + 1] = here ? P.nv : 0u;
+        ro[2 * q] = P.res_off;
+        ro[2 * q + 1] = P.res_off + nq;
+      }
+      double r[2 * PP][K];
+      residues<T, K, 2 * PP>(x, nx, r, PM, (int)((pc + T - 1) / T), S.u.st.tile);
+#pragma unroll
+      for (int o = 0; o < 2 * PP; ++o) {
+        if (b0 + o / 2 >= npairs) break;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if ((uint32_t)k * T + tid < pc) args.res[ro[o] + pb + k * T + tid] = (uint32_t)canon(r[o][k], PM[k]);
+      }
+    }
+  };
+  if (C.G == 1) {
+    residues_of_part(0);
+  } else {
+    // Every part's residue job has a state word after the group's exchange
+    // slots: 0 free, 1 taken, 2 done.  A part takes its own job, or waits
+    // for the part that took it (its results are the same either way, so
+    // after a timed-out wait it computes them itself).  Then, instead of
+    // waiting at its first exchange for the parts not yet started, it takes
+    // their jobs, which they will find done.
+    unsigned long long* job = C.bar + (size_t)2 * (C.G + 1) * B;
+    auto finish_job = [&](uint32_t p) {
+      __syncthreads();
+      if (tid == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(job + p, 2ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    };
+    if (tid == 0) {
+      unsigned long long z = 0;
+      const bool mine = __hip_atomic_compare_exchange_strong(job + C.g, &z, 1ull, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t st = mine ? 1u : 0u;
+      if (!mine) {
+        uint32_t spins = 0;
+        while (__hip_atomic_load(job + C.g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 2ull) {
+          if (++spins > C.spin_limit) {
+            st = 1u;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+      S.flag = st;
+    }
+    __syncthreads();
+    if (S.flag) {
+      residues_of_part(C.g);
+      finish_job(C.g);
+    } else {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    for (;;) {
+      __syncthreads();
+      if (wave == 0) {
+        uint32_t take = ~0u;
+        for (uint32_t p0 = 0; p0 < C.G && take == ~0u; p0 += 64) {
+          const uint32_t p = p0 + lane;
+          const bool free_job =
+              p < C.G && p != C.g && __hip_atomic_load(job + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull;
+          uint64_t m = __ballot(free_job);
+          while (m && take == ~0u) {
+            const uint32_t q = p0 + (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            uint32_t got = ~0u;
+            if (lane == 0) {
+              unsigned long long z = 0;
+              if (__hip_atomic_compare_exchange_strong(job + q, &z, 1ull, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT))
+                got = q;
+            }
+            take = __shfl(got, 0);
+          }
+        }
+        if (lane == 0) S.item = take;
+      }
+      __syncthreads();
+      const uint32_t p = S.item;
+      if (p == ~0u) break;
+      residues_of_part(p);
+      finish_job(p);
+    }
+  }
+
   ModD M[K];
   double ck[K];
 #pragma unroll
@@ -411,35 +523,6 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       M[k].minv = 1.0 / 3.0;
     }
     ck[k] = 0.5 + 2.0 * (double)(IDX_MAX - (base + j));
-  }
-
-  // ---- (a) residues of u and v, stored canonical for (c) -----------------
-  // PP pairs (2 PP operands) at a time.
-  constexpr int PP = (B >= 2 && K <= 2) ? 2 : 1;
-  for (uint32_t b0 = 0; b0 < npairs; b0 += PP) {
-    const uint32_t* x[2 * PP];
-    uint32_t nx[2 * PP];
-    uint64_t ro[2 * PP];
-#pragma unroll
-    for (int q = 0; q < PP; ++q) {
-      const bool here = b0 + q < npairs;
-      const PairDesc& P = args.pairs[Gd.first + (here ? b0 + q : b0)];
-      x[2 * q] = args.limbs + P.u_off;
-      x[2 * q + 1] = args.limbs + P.v_off;
-      nx[2 * q] = here ? P.nu : 0u;
-      nx[2 * q + 1] = here ? P.nv : 0u;
-      ro[2 * q] = P.res_off;
-      ro[2 * q + 1] = P.res_off + nq;
-    }
-    double r[2 * PP][K];
-    residues<T, K, 2 * PP>(x, nx, r, M, rows, S.u.st.tile);
-#pragma unroll
-    for (int o = 0; o < 2 * PP; ++o) {
-      if (b0 + o / 2 >= npairs) break;
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-        if ((uint32_t)k * T + tid < cnt) args.res[ro[o] + base + k * T + tid] = (uint32_t)canon(r[o][k], M[k]);
-    }
   }
 
   // ---- (b) elimination ----------------------------------------------------

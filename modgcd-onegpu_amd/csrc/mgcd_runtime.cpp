@@ -134,6 +134,91 @@ int cmp_mag(const uint32_t* a, size_t na, const uint32_t* b, size_t nb) {
   return 0;
 }
 
+// Buffers of the device conversion (grow-only).
+struct MrBufs {
+  DevBuf pairs, cum, buf0, buf1, res, lens;
+  std::vector<MrPair> h_pairs;
+  std::vector<uint32_t> h_cum, h_res, h_lens;
+  void release() {
+    for (DevBuf* d : {&pairs, &cum, &buf0, &buf1, &res, &lens}) d->release();
+  }
+};
+
+// One-pair calls convert fewer digits than this on the host (a launch
+// sequence costs more than a short Horner loop).
+constexpr uint32_t kMrHostDigits = 64;
+
+uint32_t ceil_log2(uint32_t k) {
+  uint32_t j = 0;
+  while ((1ull << j) < k) ++j;
+  return j;
+}
+
+// |G| of every (dig_off, k) listed, from the digits in device memory at
+// d_digits, by the product tree of mgcd_convert.hip on stream s; waits for
+// the stream.  out[i] receives pair i's limbs.
+hipError_t mr_convert(hipStream_t s, const mgcd_pair_t* d_digits, const std::vector<uint64_t>& dig_off,
+                      const std::vector<uint32_t>& ks, MrBufs& mb, std::vector<std::vector<uint32_t>*>& out) {
+  const uint32_t n = (uint32_t)ks.size();
+  if (n == 0) return hipSuccess;
+  mb.h_pairs.resize(n);
+  uint64_t buf = 0, res = 0;
+  uint32_t levels = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    MrPair& p = mb.h_pairs[i];
+    p.dig_off = dig_off[i];
+    p.k = ks[i];
+    p.buf_off = buf;
+    p.out_off = res;
+    p.root_level = ceil_log2(p.k);
+    buf += 8ull * p.k + 8;
+    res += (uint64_t)p.k + 1;
+    levels = std::max(levels, p.root_level);
+  }
+  // Level j -> j + 1 combines, per pair, the ceil(k / 2^(j+1)) output nodes
+  // of every pair not yet reduced to its root.
+  mb.h_cum.assign((size_t)std::max(levels, 1u) * (n + 1), 0);
+  for (uint32_t j = 0; j < levels; ++j) {
+    uint32_t* c = mb.h_cum.data() + (size_t)j * (n + 1);
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t k = ks[i];
+      const uint32_t t = (uint64_t)k > (1ull << j) ? (uint32_t)((k + (2ull << j) - 1) >> (j + 1)) : 0u;
+      c[i + 1] = c[i] + t;
+    }
+  }
+  if (mb.pairs.ensure(n * sizeof(MrPair)) != hipSuccess || mb.cum.ensure(mb.h_cum.size() * 4) != hipSuccess ||
+      mb.buf0.ensure(buf * 4) != hipSuccess || mb.buf1.ensure(buf * 4) != hipSuccess ||
+      mb.res.ensure(res * 4) != hipSuccess || mb.lens.ensure(n * 4) != hipSuccess)
+    return hipErrorOutOfMemory;
+  hipError_t e = hipMemcpyAsync(mb.pairs.p, mb.h_pairs.data(), n * sizeof(MrPair), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(mb.cum.p, mb.h_cum.data(), mb.h_cum.size() * 4, hipMemcpyHostToDevice, s);
+  uint32_t* lv[2] = {mb.buf0.as<uint32_t>(), mb.buf1.as<uint32_t>()};
+  MrArgs a{};
+  a.digits = d_digits;
+  a.pairs = mb.pairs.as<MrPair>();
+  a.npairs = n;
+  for (uint32_t j = 0; e == hipSuccess && j < levels; ++j) {
+    a.j = j;
+    a.cum = mb.cum.as<uint32_t>() + (size_t)j * (n + 1);
+    a.in = lv[j & 1];
+    a.out = lv[(j + 1) & 1];
+    e = launch_mr_level(a, mb.h_cum[(size_t)j * (n + 1) + n], s);
+  }
+  if (e == hipSuccess) e = launch_mr_emit(a, lv[0], lv[1], mb.res.as<uint32_t>(), mb.lens.as<uint32_t>(), s);
+  mb.h_res.resize(res);
+  mb.h_lens.resize(n);
+  if (e == hipSuccess) e = hipMemcpyAsync(mb.h_res.data(), mb.res.p, res * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(mb.h_lens.data(), mb.lens.p, n * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t* r = mb.h_res.data() + mb.h_pairs[i].out_off;
+    out[i]->assign(r, r + mb.h_lens[i]);
+  }
+  return hipSuccess;
+}
+
 // Signed mixed-radix digits (least significant first) -> |G| as limbs:
 // G = g_k; G = G * m_i + g_i for i = k-1 .. 1 (GmpCudaDevice-gcd.cu:243-256).
 // When G != 0, |G * m_i| >= m_i > |g_i|, so the sign never flips mid-way.
@@ -222,7 +307,9 @@ struct mgcd_ctx {
   bool batch_graph = false;  // MGCD_OPT_BATCH_GRAPH
   bool tune_loaded = false;
   uint32_t relay_min_g = 48;  // MGCD_OPT_RELAY_PARTS
+  bool device_convert = true;  // MGCD_OPT_DEVICE_CONVERT
   std::map<std::string, int> tuned;  // size class -> kernel variant
+  MrBufs mr;                   // the one-pair path's conversion buffers
 };
 
 // A prepared launch over device-resident pairs.
@@ -280,7 +367,7 @@ int plan_items(mgcd_ctx* ctx, int vi, std::vector<PairDesc>& pd, std::vector<uin
       if (gd.G > maxres) return MGCD_ERR_TOO_MANY_THREADS;
       maxG = std::max(maxG, gd.G);
       gd.bar_off = bar;
-      if (gd.G > 1) bar += 2 * ((size_t)gd.G + 1) * kv.B;
+      if (gd.G > 1) bar += 2 * ((size_t)gd.G + 1) * kv.B + gd.G;  // exchange rows, relay row, residue jobs
       groups.push_back(gd);
     }
     groups.back().count++;
@@ -508,6 +595,7 @@ int mgcd_ctx_destroy(mgcd_ctx* ctx) {
   for (DevBuf* b : {&ctx->d_mod_u32, &ctx->d_mod_d, &ctx->d_mod_inv, &ctx->limbs, &ctx->pairs, &ctx->groups, &ctx->items,
                     &ctx->queue, &ctx->bar, &ctx->digits, &ctx->trace, &ctx->res, &ctx->out})
     b->release();
+  ctx->mr.release();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -555,6 +643,9 @@ int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value) {
     case MGCD_OPT_RELAY_PARTS:
       if (value < 2) return MGCD_ERR_ARG;
       ctx->relay_min_g = (uint32_t)std::min<int64_t>(value, 1 << 30);
+      return MGCD_OK;
+    case MGCD_OPT_DEVICE_CONVERT:
+      ctx->device_convert = value != 0;
       return MGCD_OK;
     default:
       return MGCD_ERR_ARG;
@@ -644,6 +735,15 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
     HIP_TRY(hipMemcpy(pivots, ctx->trace.p, std::min<size_t>(po.ntrace, trace_cap) * 8, hipMemcpyDeviceToHost));
   if (po.status != KST_OK) return kst_to_status(po.status);
   if (po.ndig > nq) return MGCD_ERR_KERNEL;
+  if (ctx->device_convert && po.ndig > kMrHostDigits && !(digits && dcap)) {
+    std::vector<uint32_t> mag;
+    std::vector<std::vector<uint32_t>*> outs{&mag};
+    HIP_TRY(mr_convert(s, ctx->digits.as<mgcd_pair_t>(), {0}, {po.ndig}, ctx->mr, outs));
+    *ng = mag.size();
+    if (mag.size() > cap) return MGCD_ERR_CAPACITY;
+    if (!mag.empty()) std::memcpy(g, mag.data(), mag.size() * 4);
+    return MGCD_OK;
+  }
   std::vector<mgcd_pair_t> dig(po.ndig);
   if (po.ndig) HIP_TRY(hipMemcpy(dig.data(), ctx->digits.p, po.ndig * 8, hipMemcpyDeviceToHost));
   if (digits && dcap) std::memcpy(digits, dig.data(), std::min<size_t>(dcap, dig.size()) * 8);
@@ -669,6 +769,23 @@ int mgcd_gcd_limbs_trace(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint
   const uint64_t bits = bit_length(u, nu);
   if (bits == 0) return MGCD_ERR_ARG;
   return gcd_single(ctx, u, nu, v, nv, moduli_for(ctx, bits), g, cap, ng, digits, dcap, nd, pivots, tcap, nt);
+}
+
+int mgcd_mixed_radix_to_limbs(mgcd_ctx* ctx, const mgcd_pair_t* digits, size_t k, uint32_t* g, size_t cap,
+                              size_t* ng) {
+  if (!ctx || (k && !digits) || !ng || k > 0xFFFFFFFFu) return MGCD_ERR_ARG;
+  *ng = 0;
+  if (k == 0) return MGCD_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (ctx->digits.ensure(k * 8) != hipSuccess) return MGCD_ERR_NOMEM;
+  HIP_TRY(hipMemcpyAsync(ctx->digits.p, digits, k * 8, hipMemcpyHostToDevice, ctx->stream));
+  std::vector<uint32_t> mag;
+  std::vector<std::vector<uint32_t>*> outs{&mag};
+  HIP_TRY(mr_convert(ctx->stream, ctx->digits.as<mgcd_pair_t>(), {0}, {(uint32_t)k}, ctx->mr, outs));
+  *ng = mag.size();
+  if (mag.size() > cap) return MGCD_ERR_CAPACITY;
+  if (!mag.empty()) std::memcpy(g, mag.data(), mag.size() * 4);
+  return MGCD_OK;
 }
 
 int mgcd_gcd_limbs(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint32_t* g,
@@ -772,6 +889,7 @@ struct mgcd_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<uint32_t> packed;      // host copy of the kernel pairs' limbs (for retries)
+  MrBufs mr;                         // device conversion of the digits
 };
 
 namespace {
@@ -1084,9 +1202,11 @@ int mgcd_batch_run(mgcd_batch* b) {
       HIP_TRY(hipEventRecord(ctx->ev1, s));
     }
     b->h_out.resize(np);
-    b->h_digits.resize(b->ndigits);
     HIP_TRY(hipMemcpyAsync(b->h_out.data(), b->out.p, np * sizeof(PairOut), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(b->h_digits.data(), b->digits.p, b->ndigits * 8, hipMemcpyDeviceToHost, s));
+    if (!ctx->device_convert) {
+      b->h_digits.resize(b->ndigits);
+      HIP_TRY(hipMemcpyAsync(b->h_digits.data(), b->digits.p, b->ndigits * 8, hipMemcpyDeviceToHost, s));
+    }
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) b->device_s = ms * 1e-3;
@@ -1095,11 +1215,32 @@ int mgcd_batch_run(mgcd_batch* b) {
       if (!b->use_graph && hipEventElapsedTime(&ms, l.e0, l.e1) == hipSuccess) l.device_s = ms * 1e-3;
     }
     ctx->last_kernel_s = b->device_s;
-    // Digit conversion on host threads.
+    // Digit conversion: on the device (only |G| comes back), or on host
+    // threads from the copied digits.
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<size_t> todo;
     for (size_t i = 0; i < b->npairs; ++i)
       if (b->kidx[i] >= 0) todo.push_back(i);
+    if (ctx->device_convert) {
+      std::vector<uint64_t> doff;
+      std::vector<uint32_t> ks;
+      std::vector<std::vector<uint32_t>*> outs;
+      for (size_t i : todo) {
+        const PairDesc& p = b->pd[b->kidx[i]];
+        const PairOut& o = b->h_out[b->kidx[i]];
+        if (o.status != KST_OK || o.ndig > p.nq) {
+          b->status[i] = o.status != KST_OK ? kst_to_status(o.status) : MGCD_ERR_KERNEL;
+          b->host_result[i].clear();
+          continue;
+        }
+        b->status[i] = MGCD_OK;
+        doff.push_back(p.dig_off);
+        ks.push_back(o.ndig);
+        outs.push_back(&b->host_result[i]);
+      }
+      HIP_TRY(mr_convert(s, b->digits.as<mgcd_pair_t>(), doff, ks, b->mr, outs));
+    }
+
     std::atomic<size_t> next{0};
     auto work = [&]() {
       for (;;) {
@@ -1117,12 +1258,14 @@ int mgcd_batch_run(mgcd_batch* b) {
         digits_to_limbs(b->h_digits.data() + p.dig_off, o.ndig, b->host_result[i]);
       }
     };
-    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (todo.size() < 64) nth = 1;
-    std::vector<std::thread> th;
-    for (unsigned k = 1; k < nth; ++k) th.emplace_back(work);
-    work();
-    for (auto& x : th) x.join();
+    if (!ctx->device_convert) {
+      unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+      if (todo.size() < 64) nth = 1;
+      std::vector<std::thread> th;
+      for (unsigned k = 1; k < nth; ++k) th.emplace_back(work);
+      work();
+      for (auto& x : th) x.join();
+    }
     b->host_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     // Pairs the reference's sizing left short of moduli: retry one by one
     // with a quarter more moduli each time (MGCD_OPT_RETRY).  A pair whose
@@ -1241,6 +1384,7 @@ int mgcd_batch_destroy(mgcd_batch* b) {
   for (DevBuf* d : {&b->limbs, &b->pairs, &b->groups_d, &b->items_d, &b->queue, &b->bar, &b->digits, &b->trace, &b->res,
                     &b->out})
     d->release();
+  b->mr.release();
   delete b;
   return MGCD_OK;
 }

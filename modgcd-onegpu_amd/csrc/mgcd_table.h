@@ -12,6 +12,12 @@
 // unsigned __int128 instead of GMP, the count is not capped at 65,536, and
 // the result is written to a binary file that the runtime loads.
 //
+// An "all primes" table (CW section 2.4) keeps every L-bit prime in the
+// same descending walk: this build's FP64 arithmetic is exact for any odd
+// modulus below 2^32 and needs no division criterion, so the filter only
+// keeps tables (and pivot sequences) equal to the reference's; without it,
+// L = 22 has 140,336 moduli instead of 97,358.
+//
 // File layout (little endian):
 //   MgcdTableHeader (48 bytes) followed by uint32_t moduli[count].
 #pragma once
@@ -28,7 +34,7 @@ struct MgcdTableHeader {
   char magic[8];        // "MGCDMODS"
   uint32_t version;     // 1
   uint32_t L;           // modulus width in bits (every modulus is in [2^(L-1), 2^L))
-  uint32_t W;           // 64: word width of the division criterion
+  uint32_t W;           // 64: word width of the division criterion; 0: every prime (no criterion)
   uint32_t count;       // number of moduli that follow
   uint64_t checksum;    // FNV-1a 64 over the moduli array bytes
   uint64_t reserved[2];
@@ -57,9 +63,10 @@ inline bool cwUsable(uint32_t D, int L, int W = 64) {
   return (Ncr >> W) != 0;                              // sizeinbase(Ncr,2) > W
 }
 
-// Generate up to `count` usable L-bit primes in descending order.
-// Returns the moduli found (fewer than `count` if the range is exhausted).
-inline std::vector<uint32_t> generateModuli(int L, size_t count) {
+// Generate up to `count` usable L-bit primes in descending order (every
+// prime when all_primes).  Returns the moduli found (fewer than `count` if
+// the range is exhausted).
+inline std::vector<uint32_t> generateModuli(int L, size_t count, bool all_primes = false) {
   std::vector<uint32_t> out;
   if (L < 3 || L > 32 || count == 0) return out;
   const uint64_t top = (uint64_t(1) << L) - 1;          // 2^L - 1 (odd)
@@ -95,7 +102,7 @@ inline std::vector<uint32_t> generateModuli(int L, size_t count) {
     for (uint64_t k = n; k-- > 0 && out.size() < count;) {
       if (comp[k]) continue;
       const uint32_t D = uint32_t(lo + 2 * k);
-      if (cwUsable(D, L)) out.push_back(D);
+      if (all_primes || cwUsable(D, L)) out.push_back(D);
     }
     if (lo <= (bottom | 1)) break;
     hi = lo - 2;
@@ -103,13 +110,13 @@ inline std::vector<uint32_t> generateModuli(int L, size_t count) {
   return out;
 }
 
-inline bool writeTable(const std::string& path, int L, const std::vector<uint32_t>& moduli) {
+inline bool writeTable(const std::string& path, int L, const std::vector<uint32_t>& moduli, bool all_primes = false) {
   MgcdTableHeader h;
   std::memset(&h, 0, sizeof h);
   std::memcpy(h.magic, "MGCDMODS", 8);
   h.version = 1;
   h.L = uint32_t(L);
-  h.W = 64;
+  h.W = all_primes ? 0 : 64;
   h.count = uint32_t(moduli.size());
   h.checksum = fnv1a64(moduli.data(), moduli.size() * sizeof(uint32_t));
   FILE* f = std::fopen(path.c_str(), "wb");
@@ -128,7 +135,7 @@ inline std::string readTable(const std::string& path, int* L, std::vector<uint32
   MgcdTableHeader h;
   if (std::fread(&h, sizeof h, 1, f) != 1) { std::fclose(f); return "short moduli table header in " + path; }
   if (std::memcmp(h.magic, "MGCDMODS", 8) != 0 || h.version != 1) { std::fclose(f); return "bad moduli table magic/version in " + path; }
-  if (h.L < 3 || h.L > 32 || h.W != 64) { std::fclose(f); return "bad moduli width in " + path; }
+  if (h.L < 3 || h.L > 32 || (h.W != 64 && h.W != 0)) { std::fclose(f); return "bad moduli width in " + path; }
   moduli->resize(h.count);
   size_t got = std::fread(moduli->data(), sizeof(uint32_t), h.count, f);
   std::fclose(f);

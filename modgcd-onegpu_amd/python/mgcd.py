@@ -38,7 +38,7 @@ EXPORTS = [
     "mgcd_certify_quasi_quotient", "mgcd_last_error_detail", "mgcd_ctx_last_kernel_seconds",
     "mgcd_ctx_synchronize", "mgcd_batch_pair_stats", "mgcd_ctx_set_option", "mgcd_batch_launch_count",
     "mgcd_batch_launch_info", "mgcd_autotune", "mgcd_server_create", "mgcd_server_submit", "mgcd_server_wait",
-    "mgcd_server_stats", "mgcd_server_destroy",
+    "mgcd_server_stats", "mgcd_server_destroy", "mgcd_mixed_radix_to_limbs",
 ]
 
 OPT_FORCE_MODULI = 1
@@ -48,6 +48,7 @@ OPT_SINGLE_SLOTS = 4
 OPT_AUTOTUNE = 5
 OPT_BATCH_GRAPH = 6
 OPT_RELAY_PARTS = 7
+OPT_DEVICE_CONVERT = 8
 
 
 class Pair(ctypes.Structure):
@@ -118,6 +119,7 @@ def lib():
         L.mgcd_server_stats.argtypes = [ctypes.c_void_p, P64, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), P64]
         L.mgcd_server_destroy.argtypes = [ctypes.c_void_p]
+        L.mgcd_mixed_radix_to_limbs.argtypes = [ctypes.c_void_p, PPair, ctypes.c_size_t, P32, ctypes.c_size_t, PSZ]
         _lib = L
     return _lib
 
@@ -154,12 +156,15 @@ def device_count():
 class Context:
     """A device with one moduli table (L in 22, 27, 32) resident in HBM."""
 
-    def __init__(self, L=32, device=0, table_path=None):
+    def __init__(self, L=32, device=0, table_path=None, all_primes=False):
+        """all_primes: the table of every L-bit prime (moduli_L<L>_all.bin)
+        instead of the Cavagnino-Werbrouck-filtered one the reference uses."""
         self.L = L
         h = ctypes.c_void_p()
         path = table_path.encode() if table_path else None
-        if path is None and "MGCD_TABLE_DIR" not in os.environ:
-            path = os.path.join(DATA_DIR, f"moduli_L{L}.bin").encode()
+        if path is None and (all_primes or "MGCD_TABLE_DIR" not in os.environ):
+            d = os.environ.get("MGCD_TABLE_DIR") or DATA_DIR
+            path = os.path.join(d, f"moduli_L{L}{'_all' if all_primes else ''}.bin").encode()
         check(lib().mgcd_ctx_create(ctypes.byref(h), device, L, path), "mgcd_ctx_create")
         self.h = h
 
@@ -222,6 +227,21 @@ class Context:
         d = [(digits[i].modulus, digits[i].value) for i in range(min(nd.value, dcap))]
         t = [(pivots[i].modulus, pivots[i].value) for i in range(min(nt.value, tcap))]
         return st, (limbs_to_int(g[:ng.value]) if st == OK else None), d, t
+
+    def mixed_radix(self, digits):
+        """|G| of signed mixed-radix digits [(m_0, g_0), (m_1, g_1), ...] (least
+        significant first), converted on the device by the product tree."""
+        k = len(digits)
+        arr = (Pair * max(k, 1))()
+        for i, (m, g) in enumerate(digits):
+            arr[i].modulus = m
+            arr[i].value = g
+        cap = k + 2
+        out = np.zeros(cap, np.uint32)
+        ng = ctypes.c_size_t()
+        check(lib().mgcd_mixed_radix_to_limbs(self.h, arr, k, _p32(out), cap, ctypes.byref(ng)),
+              "mgcd_mixed_radix_to_limbs")
+        return limbs_to_int(out[:ng.value])
 
     def set_option(self, option, value):
         check(lib().mgcd_ctx_set_option(self.h, option, value), "mgcd_ctx_set_option")

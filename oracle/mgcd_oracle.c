@@ -25,6 +25,10 @@
  *  - oracle_gcd_kernel: pivots checked against a brute-force big-integer
  *    Weber elimination on small inputs (exact U, V as Python ints); digits
  *    reconstruct the gcd; final result equals mpz_gcd and math.gcd.
+ *  - oracle_gcd_kernel_exact (the same steps with x % m, for "all primes"
+ *    tables): equal to oracle_gcd_kernel on CW tables; over all-primes
+ *    moduli (some outside the CW condition), pivots and digits equal the
+ *    brute-force run.
  *  - oracle_num_moduli_needed: the reference's float formula; closed-form
  *    values at the configs' sizes.
  */
@@ -108,8 +112,12 @@ uint64_t oracle_cw_inverse(uint32_t m, int L) {
   return 1 + (q << L) + (r << L) / D;
 }
 
-/* mod (getGcdKernel.cu:248-251): x - m * (umulhi64(inv, x) >> (L-1)). */
+/* mod (getGcdKernel.cu:248-251): x - m * (umulhi64(inv, x) >> (L-1)).
+ * inverse == 0 selects exact reduction (x % m): the reference's reduction
+ * is exact only for moduli that satisfy the Cavagnino-Werbrouck condition,
+ * and "all primes" tables hold others. */
 static inline uint32_t mod64(uint64_t x, const oracle_modulus_t* m) {
+  if (m->inverse == 0) return (uint32_t)(x % m->modulus);
   uint64_t hi = (uint64_t)(((unsigned __int128)m->inverse * x) >> 64);
   return (uint32_t)(x - (uint64_t)m->modulus * (hi >> (m->L - 1)));
 }
@@ -159,11 +167,14 @@ static inline uint32_t modDiv(uint32_t u, uint32_t v, const oracle_modulus_t* m)
 /* modMP (getGcdKernel.cu:297-324): Horner over 32-bit limbs, most significant
  * first (the reference stages 32 limbs at a time in shared memory; the order
  * of the mod steps is the same). */
+static uint32_t mod_mp(const uint32_t* x, size_t n, const oracle_modulus_t* q) {
+  uint64_t r = 0;
+  for (size_t i = n; i-- != 0;) r = mod64(r << 32 | x[i], q);
+  return (uint32_t)r;
+}
 uint32_t oracle_mod_mp(const uint32_t* x, size_t n, uint32_t m, int L) {
   oracle_modulus_t q = {m, oracle_cw_inverse(m, L), L};
-  uint64_t r = 0;
-  for (size_t i = n; i-- != 0;) r = mod64(r << 32 | x[i], &q);
-  return (uint32_t)r;
+  return mod_mp(x, n, &q);
 }
 
 /* ---------------------------------------------------------------------------
@@ -201,8 +212,8 @@ static inline int ffs_abs(int32_t x) {
   return a ? __builtin_ctz(a) + 1 : 0;
 }
 
-int oracle_gcd_kernel(const uint32_t* u, size_t nu, const uint32_t* v, size_t nv,
-                      const uint32_t* moduli, int nq, int L,
+static int gcd_kernel(const uint32_t* u, size_t nu, const uint32_t* v, size_t nv,
+                      const uint32_t* moduli, int nq, int L, int exact,
                       oracle_pair_t* digits, size_t dcap, size_t* ndigits,
                       oracle_pair_t* trace, size_t tcap, size_t* ntrace) {
   if (nq <= 0 || L < 3 || L > 32) return ORACLE_ARG;
@@ -221,10 +232,10 @@ int oracle_gcd_kernel(const uint32_t* u, size_t nu, const uint32_t* v, size_t nv
 
   for (int i = 0; i < nq; ++i) {                 /* MGCD1, MGCD2 (:355-361) */
     q[i].modulus = moduli[i];
-    q[i].inverse = oracle_cw_inverse(moduli[i], L);
+    q[i].inverse = exact ? 0 : oracle_cw_inverse(moduli[i], L);
     q[i].L = L;
-    uq[i] = oracle_mod_mp(u, nu, moduli[i], L);
-    vq[i] = oracle_mod_mp(v, nv, moduli[i], L);
+    uq[i] = mod_mp(u, nu, &q[i]);
+    vq[i] = mod_mp(v, nv, &q[i]);
     active[i] = 1;
     my[i].modulus = moduli[i];                   /* :368-369 */
     my[i].value = (vq[i] == 0) ? MOD_INFINITY : toSigned(modDiv(uq[i], vq[i], &q[i]), &q[i]);
@@ -293,6 +304,23 @@ done:
   if (status == ORACLE_OK && nd > dcap) status = ORACLE_CAPACITY;
   free(q); free(uq); free(vq); free(tq); free(active); free(my);
   return status;
+}
+
+/* The reference kernel's steps with its reduction (CW moduli). */
+int oracle_gcd_kernel(const uint32_t* u, size_t nu, const uint32_t* v, size_t nv,
+                      const uint32_t* moduli, int nq, int L,
+                      oracle_pair_t* digits, size_t dcap, size_t* ndigits,
+                      oracle_pair_t* trace, size_t tcap, size_t* ntrace) {
+  return gcd_kernel(u, nu, v, nv, moduli, nq, L, 0, digits, dcap, ndigits, trace, tcap, ntrace);
+}
+/* The same steps with exact reduction: any odd moduli ("all primes" tables).
+ * Equal to oracle_gcd_kernel wherever every modulus satisfies the CW
+ * condition. */
+int oracle_gcd_kernel_exact(const uint32_t* u, size_t nu, const uint32_t* v, size_t nv,
+                            const uint32_t* moduli, int nq, int L,
+                            oracle_pair_t* digits, size_t dcap, size_t* ndigits,
+                            oracle_pair_t* trace, size_t tcap, size_t* ntrace) {
+  return gcd_kernel(u, nu, v, nv, moduli, nq, L, 1, digits, dcap, ndigits, trace, tcap, ntrace);
 }
 
 /* ---------------------------------------------------------------------------

@@ -50,6 +50,8 @@ def lib():
         L.oracle_gcd_kernel.argtypes = [u32p, ctypes.c_size_t, u32p, ctypes.c_size_t, u32p, ctypes.c_int,
                                         ctypes.c_int, pp, ctypes.c_size_t, szp, pp, ctypes.c_size_t, szp]
         L.oracle_gcd_kernel.restype = ctypes.c_int
+        L.oracle_gcd_kernel_exact.argtypes = L.oracle_gcd_kernel.argtypes
+        L.oracle_gcd_kernel_exact.restype = ctypes.c_int
         L.oracle_horner.argtypes = [pp, ctypes.c_size_t, u32p, ctypes.c_size_t]
         L.oracle_horner.restype = ctypes.c_size_t
         L.oracle_mpz_gcd.argtypes = [u32p, ctypes.c_size_t, u32p, ctypes.c_size_t, u32p, ctypes.c_size_t]
@@ -98,9 +100,11 @@ def mod_mp(x, m, L):
     return lib().oracle_mod_mp(p, len(a), m, L)
 
 
-def gcd_kernel(u, v, moduli, L, dcap=None, tcap=None):
+def gcd_kernel(u, v, moduli, L, dcap=None, tcap=None, exact=False):
     """Runs the kernel restatement.  u, v: Python ints (the limb arrays passed
     are |u|, |v| exactly as the reference's mpz_export would stage them).
+    exact: reduce with x % m instead of the reference's division by multiply
+    (required for moduli outside the Cavagnino-Werbrouck condition).
     Returns (status, digits[(m, g)], pivots[(p, b)])."""
     ua, up = _u32(int_to_limbs(u))
     va, vp = _u32(int_to_limbs(v))
@@ -112,8 +116,8 @@ def gcd_kernel(u, v, moduli, L, dcap=None, tcap=None):
     trace = (Pair * tcap)()
     nd = ctypes.c_size_t()
     nt = ctypes.c_size_t()
-    st = lib().oracle_gcd_kernel(up, len(ua), vp, len(va), mp, nq, L, digits, dcap, ctypes.byref(nd),
-                                 trace, tcap, ctypes.byref(nt))
+    fn = lib().oracle_gcd_kernel_exact if exact else lib().oracle_gcd_kernel
+    st = fn(up, len(ua), vp, len(va), mp, nq, L, digits, dcap, ctypes.byref(nd), trace, tcap, ctypes.byref(nt))
     d = [(digits[i].modulus, digits[i].value) for i in range(min(nd.value, dcap))]
     t = [(trace[i].modulus, trace[i].value) for i in range(min(nt.value, tcap))]
     return st, d, t

@@ -73,14 +73,39 @@ def test_messages_are_the_references(mgcd):
         assert L.mgcd_strerror(code).decode() == msg
 
 
-def read_table(L):
-    raw = open(os.path.join(PKG, "data", f"moduli_L{L}.bin"), "rb").read()
+def read_table(L, all_primes=False, path=None):
+    path = path or os.path.join(PKG, "data", f"moduli_L{L}{'_all' if all_primes else ''}.bin")
+    raw = open(path, "rb").read()
     assert raw[:8] == b"MGCDMODS"
     version, width, W, count = np.frombuffer(raw[8:24], np.uint32)
     checksum = int(np.frombuffer(raw[24:32], np.uint64)[0])
     mods = np.frombuffer(raw[48:], np.uint32)
-    assert (version, width, W) == (1, L, 64) and count == len(mods)
+    assert (version, width, W) == (1, L, 0 if all_primes else 64) and count == len(mods)
     return mods, checksum
+
+
+def is_prime32(n):
+    # deterministic Miller-Rabin below 2^32 (bases 2, 7, 61)
+    if n < 2:
+        return False
+    for p in (2, 3, 5, 7, 11, 13, 61):
+        if n % p == 0:
+            return n == p
+    d, r = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        r += 1
+    for a in (2, 7, 61):
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(r - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
 
 
 def fnv1a64(arr):
@@ -108,6 +133,35 @@ def test_tables_match_oracle_and_checksums(oracle, L, count):
     head = 20000 if L != 22 else count
     assert np.array_equal(mods[:head], oracle.create_moduli(L, head))
     assert np.all(mods[:-1] > mods[1:])
+
+
+@pytest.mark.parametrize("L,count", [(22, 140336), (27, 262144), (32, 262144)])
+def test_all_primes_tables(L, count):
+    # Every L-bit prime, descending (CW section 2.4): L = 22 has 140,336.
+    mods, checksum = read_table(L, all_primes=True)
+    assert len(mods) == count and fnv1a64(mods) == checksum
+    assert np.all(mods[:-1] > mods[1:])
+    assert all(is_prime32(int(m)) for m in mods[:: 97])
+    # complete at the top: no prime above the 2000th entry is missing
+    head = set(int(m) for m in mods[:2001])
+    for x in range((1 << L) - 1, int(mods[2000]), -2):
+        assert (x in head) == is_prime32(x), x
+    # the filtered table is a subsequence of it
+    cw, _ = read_table(L)
+    assert np.isin(cw[:20000], mods).all()
+    if L == 22:
+        assert int(mods[-1]) > (1 << 21) and np.isin(cw, mods).all()
+
+
+def test_all_primes_generator_exhaustive_small_width(tmp_path):
+    # At L = 14 the generator must list exactly the primes in (2^13, 2^14), descending.
+    exe = os.path.join(PKG, "bin", "createModuli")
+    out = tmp_path / "m14.bin"
+    r = subprocess.run([exe, "14", "100000", str(out), "--all-primes"], capture_output=True, text=True)
+    assert r.returncode == 2  # fewer than asked for exist
+    mods, _ = read_table(14, all_primes=True, path=str(out))
+    want = [x for x in range((1 << 14) - 1, 1 << 13, -1) if is_prime32(x)]
+    assert mods.tolist() == want
 
 
 def test_reference_createModuli_matches_table_when_built():

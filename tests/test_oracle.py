@@ -194,6 +194,41 @@ def test_kernel_matches_bruteforce(oracle, L, bits, gbits, seed):
     assert oracle.horner(digits) == math.gcd(u, v) == oracle.mpz_gcd(u, v)
 
 
+def _top_primes(L, n):
+    out, x = [], (1 << L) - 1
+    while len(out) < n:
+        if is_prime32(x):
+            out.append(x)
+        x -= 2
+    return out
+
+
+@pytest.mark.parametrize("L,bits,gbits,seed", [(32, 2000, 300, 12), (27, 1500, 700, 13), (22, 1200, 64, 14)])
+def test_exact_kernel_equals_reference_kernel_on_cw_tables(oracle, L, bits, gbits, seed):
+    rnd = random.Random(seed)
+    u, v = _small_pair(rnd, bits, gbits)
+    nq = oracle.num_moduli_needed(u.bit_length(), L)
+    mods = np.array(oracle.create_moduli(L, nq), np.uint32)
+    assert oracle.gcd_kernel(u, v, mods, L, exact=True) == oracle.gcd_kernel(u, v, mods, L)
+
+
+@pytest.mark.parametrize("L,bits,gbits,seed", [(32, 2000, 300, 15), (27, 1500, 700, 16), (22, 1200, 64, 17)])
+def test_exact_kernel_matches_bruteforce_over_all_primes(oracle, L, bits, gbits, seed):
+    # "All primes" moduli (no CW filter): the exact-reduction kernel must
+    # still follow the brute-force Weber run step by step.
+    rnd = random.Random(seed)
+    u, v = _small_pair(rnd, bits, gbits)
+    nq = oracle.num_moduli_needed(u.bit_length(), L)
+    mods = _top_primes(L, nq)
+    assert any(not cw_usable(m, L) for m in mods)
+    st, digits, pivots = oracle.gcd_kernel(u, v, np.array(mods, np.uint32), L, exact=True)
+    bp, bd = wb.weber(u, v, mods)
+    assert st == oracle.OK
+    assert pivots == bp
+    assert digits == bd
+    assert oracle.horner(digits) == math.gcd(u, v)
+
+
 def test_kernel_degenerate_and_failures(oracle):
     L = 32
     mods = oracle.create_moduli(L, 4096)
