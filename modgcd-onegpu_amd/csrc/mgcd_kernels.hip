@@ -58,7 +58,6 @@ constexpr double TINF = 2147483648.0;    // vq == 0: the reference's MOD_INFINIT
 constexpr double TZERO = -2147483648.0;  // uq == vq == 0
 constexpr double TRET = 4294967296.0;    // retired modulus
 constexpr double TFIN = 2147483648.0;    // |t| < TFIN: finite
-constexpr double ZOK = 0x1p40;           // within an update: z = (t - b) + ZOK marks "t' = p'/(t - b)"
 
 // Keys are doubles holding exact integers, compared with v_min_f64:
 //   elimination  |t| * 2^20 + 2 (IDX_MAX - idx) + [t < 0]
@@ -103,13 +102,18 @@ __device__ __forceinline__ int ffs_abs(int32_t x) {
   return a ? __builtin_ctz(a) + 1 : 0;
 }
 
-// Reads x[i] after `after` has been computed: the empty asm makes the
+// Reads p[i] after `after` has been computed: the empty asm makes the
 // address depend on it, so the compiler cannot hoist the load (and keep
-// the value in registers across a long stretch of code).
+// the value in registers across a long stretch of code).  The cursor is
+// updated in place, so a chain of such reads keeps one address register.
 template <class V>
-__device__ __forceinline__ V load_after(const V* x, int i, double after) {
-  typedef const __attribute__((address_space(3))) V* LdsPtr;
-  LdsPtr p = (LdsPtr)x;
+using LdsPtr = const __attribute__((address_space(3))) V*;
+template <class V>
+__device__ __forceinline__ LdsPtr<V> lds_ptr(const V* x) {
+  return (LdsPtr<V>)x;
+}
+template <class V>
+__device__ __forceinline__ V load_after(LdsPtr<V>& p, int i, double after) {
   asm volatile("" : "+v"(p) : "v"(after));
   return p[i];
 }
@@ -655,40 +659,38 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
     // values to invert (1 where there is nothing to invert), one inverse,
     // then back down (Montgomery's batch inversion).
     wkey = KNONE;
+    LdsPtr<double> bbp = lds_ptr(S.bb), ppp = lds_ptr(S.pp);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (k >= rows) break;
       // Forward: the value to invert is y = t - b where that is invertible
-      // (|y| < 2^33), else 1.  t is replaced by z = y + ZOK there, and by its
-      // final value elsewhere: t - b == 0 gives infinity, infinity gives 0,
-      // retired and zero states stay.
-      double P[B], key[B];
+      // (a finite t with t != b mod m), else 1; prefix products of the y.
+      double P[B], Y[B], key[B];
+      bool okv[B];
 #pragma unroll
       for (int b = 0; b < B; ++b) {
         // Branch-free: every candidate is computed, then selected.
         const double tv = t[b][k];
-        const double d = tv - load_after(S.bb, b, b == 0 ? tv : P[b - 1]);
-        const bool fin = __builtin_fabs(tv) < TFIN;
-        const bool ok = fin & (d != 0.0) & (__builtin_fabs(d) != M[k].m);
-        // Every value alt can take (0, TINF and the sentinels TZERO, TRET a
-        // non-finite t holds) has a zero low word: the selects work on the
-        // high words only.
-        const uint32_t alt_hi = tv == TINF ? 0u : (fin ? hi32(TINF) : hi32(tv));
-        const double dz = d + ZOK;
-        const double y = mk64(ok ? lo32(d) : 0u, ok ? hi32(d) : hi32(1.0));
-        t[b][k] = mk64(ok ? lo32(dz) : 0u, ok ? hi32(dz) : alt_hi);
-        P[b] = b == 0 ? y : A::mulmod(P[b - 1], y, M[k]);
+        const double d = tv - load_after(bbp, b, b == 0 ? tv : P[b - 1]);
+        okv[b] = (__builtin_fabs(tv) < TFIN) & (d != 0.0) & (__builtin_fabs(d) != M[k].m);
+        Y[b] = mk64(okv[b] ? lo32(d) : 0u, okv[b] ? hi32(d) : hi32(1.0));
+        P[b] = b == 0 ? Y[b] : A::mulmod(P[b - 1], Y[b], M[k]);
       }
       double r = A::modinv(canon(P[B - 1], M[k]), M[k]);
+      // Backward: t' = p' / y where y was invertible; elsewhere t - b == 0
+      // gives infinity, infinity gives 0, retired and zero states stay.
 #pragma unroll
       for (int b = B - 1; b >= 0; --b) {
-        const double z = t[b][k];
-        const bool ok = z > 0.5 * ZOK;
-        const double zy = z - ZOK;
+        const double tv = t[b][k];
         const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
-        if (b > 0) r = A::mulmod(r, mk64(ok ? lo32(zy) : 0u, ok ? hi32(zy) : hi32(1.0)), M[k]);
-        const double tn = tsigned(A::mulmod(load_after(S.pp, b, inv), inv, M[k]), M[k]);
-        const double tv2 = ok ? tn : z;
+        if (b > 0) r = A::mulmod(r, Y[b], M[k]);
+        const double tn = tsigned(A::mulmod(load_after(ppp, b, inv), inv, M[k]), M[k]);
+        // Every value a non-invertible t can take next (0, TINF and the
+        // sentinels TZERO, TRET) has a zero low word: the selects work on
+        // the high words only.
+        const bool fin = __builtin_fabs(tv) < TFIN;
+        const uint32_t alt_hi = tv == TINF ? 0u : (fin ? hi32(TINF) : hi32(tv));
+        const double tv2 = mk64(okv[b] ? lo32(tn) : 0u, okv[b] ? hi32(tn) : alt_hi);
         t[b][k] = tv2;
         key[b] = elim_key(tv2, ck[k]);
       }
