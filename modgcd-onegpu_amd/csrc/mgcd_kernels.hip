@@ -39,6 +39,8 @@
 // any other, so the row it overwrites has been read by everyone.  Waits
 // are bounded by spin_limit polls and then report a timeout instead of
 // hanging.
+#include <type_traits>
+
 #include "mgcd_arith.h"
 #include "mgcd_kernels.h"
 
@@ -665,17 +667,27 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       if (k >= rows) break;
       // Forward: the value to invert is y = t - b where that is invertible
       // (a finite t with t != b mod m), else 1; prefix products of the y.
+      // t == b (mod m) is rare (about one value in m), so the first pass
+      // only tests that t is finite: a y == 0 (mod m) then makes the product
+      // 0 (mod m), and the wave redoes the pass with the full test.
       double P[B], Y[B], key[B];
       bool okv[B];
+      auto forward = [&](auto full) {
 #pragma unroll
-      for (int b = 0; b < B; ++b) {
-        // Branch-free: every candidate is computed, then selected.
-        const double tv = t[b][k];
-        const double d = tv - load_after(bbp, b, b == 0 ? tv : P[b - 1]);
-        okv[b] = (__builtin_fabs(tv) < TFIN) & (d != 0.0) & (__builtin_fabs(d) != M[k].m);
-        Y[b] = mk64(okv[b] ? lo32(d) : 0u, okv[b] ? hi32(d) : hi32(1.0));
-        P[b] = b == 0 ? Y[b] : A::mulmod(P[b - 1], Y[b], M[k]);
-      }
+        for (int b = 0; b < B; ++b) {
+          // Branch-free: every candidate is computed, then selected.
+          const double tv = t[b][k];
+          const double d = tv - load_after(bbp, b, b == 0 ? tv : P[b - 1]);
+          bool ok = __builtin_fabs(tv) < TFIN;
+          if constexpr (decltype(full)::value) ok = ok & (d != 0.0) & (__builtin_fabs(d) != M[k].m);
+          okv[b] = ok;
+          Y[b] = mk64(ok ? lo32(d) : 0u, ok ? hi32(d) : hi32(1.0));
+          P[b] = b == 0 ? Y[b] : A::mulmod(P[b - 1], Y[b], M[k]);
+        }
+      };
+      forward(std::false_type{});
+      // P[B-1] is a reduced product (|P| < m) or, for B == 1, y itself (|y| <= m + 1).
+      if (__ballot((P[B - 1] == 0.0) | (__builtin_fabs(P[B - 1]) == M[k].m))) forward(std::true_type{});
       double r = A::modinv(canon(P[B - 1], M[k]), M[k]);
       // Backward: t' = p' / y where y was invertible; elsewhere t - b == 0
       // gives infinity, infinity gives 0, retired and zero states stay.
