@@ -110,7 +110,12 @@ enum {
    * the device, by a product tree (batches; one-pair calls above 64 digits),
    * and only |G| is copied back.  0: the digits are copied back and
    * converted by the host Horner loop (GmpCudaDevice-gcd.cu:113-129). */
-  MGCD_OPT_DEVICE_CONVERT = 8
+  MGCD_OPT_DEVICE_CONVERT = 8,
+  /* Polls before an in-kernel wait (cross-workgroup exchange, residue job)
+   * gives up and the pair reports MGCD_ERR_TIMEOUT: default 2^24, at least
+   * 1.  Every wait is bounded by it, so a small value makes the kernel end
+   * early with timeouts instead of results (used to test that path). */
+  MGCD_OPT_SPIN_LIMIT = 9
 };
 int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value);
 /* Signed mixed-radix digits (least significant first, the layout the kernel

@@ -156,11 +156,11 @@ uint32_t ceil_log2(uint32_t k) {
 
 // |G| of every (dig_off, k) listed, from the digits in device memory at
 // d_digits, by the product tree of mgcd_convert.hip on stream s; waits for
-// the stream.  out[i] receives pair i's limbs.
-hipError_t mr_convert(hipStream_t s, const mgcd_pair_t* d_digits, const std::vector<uint64_t>& dig_off,
-                      const std::vector<uint32_t>& ks, MrBufs& mb, std::vector<std::vector<uint32_t>*>& out) {
+// the stream.  out[i] receives pair i's limbs.  Returns an MGCD_ status.
+int mr_convert(hipStream_t s, const mgcd_pair_t* d_digits, const std::vector<uint64_t>& dig_off,
+               const std::vector<uint32_t>& ks, MrBufs& mb, std::vector<std::vector<uint32_t>*>& out) {
   const uint32_t n = (uint32_t)ks.size();
-  if (n == 0) return hipSuccess;
+  if (n == 0) return MGCD_OK;
   mb.h_pairs.resize(n);
   uint64_t buf = 0, res = 0;
   uint32_t levels = 0;
@@ -189,34 +189,32 @@ hipError_t mr_convert(hipStream_t s, const mgcd_pair_t* d_digits, const std::vec
   if (mb.pairs.ensure(n * sizeof(MrPair)) != hipSuccess || mb.cum.ensure(mb.h_cum.size() * 4) != hipSuccess ||
       mb.buf0.ensure(buf * 4) != hipSuccess || mb.buf1.ensure(buf * 4) != hipSuccess ||
       mb.res.ensure(res * 4) != hipSuccess || mb.lens.ensure(n * 4) != hipSuccess)
-    return hipErrorOutOfMemory;
-  hipError_t e = hipMemcpyAsync(mb.pairs.p, mb.h_pairs.data(), n * sizeof(MrPair), hipMemcpyHostToDevice, s);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(mb.cum.p, mb.h_cum.data(), mb.h_cum.size() * 4, hipMemcpyHostToDevice, s);
+    return MGCD_ERR_NOMEM;
+  HIP_TRY(hipMemcpyAsync(mb.pairs.p, mb.h_pairs.data(), n * sizeof(MrPair), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(mb.cum.p, mb.h_cum.data(), mb.h_cum.size() * 4, hipMemcpyHostToDevice, s));
   uint32_t* lv[2] = {mb.buf0.as<uint32_t>(), mb.buf1.as<uint32_t>()};
   MrArgs a{};
   a.digits = d_digits;
   a.pairs = mb.pairs.as<MrPair>();
   a.npairs = n;
-  for (uint32_t j = 0; e == hipSuccess && j < levels; ++j) {
+  for (uint32_t j = 0; j < levels; ++j) {
     a.j = j;
     a.cum = mb.cum.as<uint32_t>() + (size_t)j * (n + 1);
     a.in = lv[j & 1];
     a.out = lv[(j + 1) & 1];
-    e = launch_mr_level(a, mb.h_cum[(size_t)j * (n + 1) + n], s);
+    HIP_TRY(launch_mr_level(a, mb.h_cum[(size_t)j * (n + 1) + n], s));
   }
-  if (e == hipSuccess) e = launch_mr_emit(a, lv[0], lv[1], mb.res.as<uint32_t>(), mb.lens.as<uint32_t>(), s);
+  HIP_TRY(launch_mr_emit(a, lv[0], lv[1], mb.res.as<uint32_t>(), mb.lens.as<uint32_t>(), s));
   mb.h_res.resize(res);
   mb.h_lens.resize(n);
-  if (e == hipSuccess) e = hipMemcpyAsync(mb.h_res.data(), mb.res.p, res * 4, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(mb.h_lens.data(), mb.lens.p, n * 4, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return e;
+  HIP_TRY(hipMemcpyAsync(mb.h_res.data(), mb.res.p, res * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(mb.h_lens.data(), mb.lens.p, n * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   for (uint32_t i = 0; i < n; ++i) {
     const uint32_t* r = mb.h_res.data() + mb.h_pairs[i].out_off;
     out[i]->assign(r, r + mb.h_lens[i]);
   }
-  return hipSuccess;
+  return MGCD_OK;
 }
 
 // Signed mixed-radix digits (least significant first) -> |G| as limbs:
@@ -308,6 +306,7 @@ struct mgcd_ctx {
   bool tune_loaded = false;
   uint32_t relay_min_g = 48;  // MGCD_OPT_RELAY_PARTS
   bool device_convert = true;  // MGCD_OPT_DEVICE_CONVERT
+  uint32_t spin_limit = kSpinLimit;  // MGCD_OPT_SPIN_LIMIT
   std::map<std::string, int> tuned;  // size class -> kernel variant
   MrBufs mr;                   // the one-pair path's conversion buffers
 };
@@ -647,6 +646,10 @@ int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value) {
     case MGCD_OPT_DEVICE_CONVERT:
       ctx->device_convert = value != 0;
       return MGCD_OK;
+    case MGCD_OPT_SPIN_LIMIT:
+      if (value < 1 || value > (int64_t)1 << 31) return MGCD_ERR_ARG;
+      ctx->spin_limit = (uint32_t)value;
+      return MGCD_OK;
     default:
       return MGCD_ERR_ARG;
   }
@@ -720,7 +723,7 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
   ka.trace = ctx->trace.as<mgcd_pair_t>();
   ka.res = ctx->res.as<uint32_t>();
   ka.out = ctx->out.as<PairOut>();
-  ka.spin_limit = kSpinLimit;
+  ka.spin_limit = ctx->spin_limit;
   ka.relay_min_g = ctx->relay_min_g;
   rc = launch(ctx, ls, ka);
   if (rc != MGCD_OK) return rc;
@@ -738,7 +741,8 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
   if (ctx->device_convert && po.ndig > kMrHostDigits && !(digits && dcap)) {
     std::vector<uint32_t> mag;
     std::vector<std::vector<uint32_t>*> outs{&mag};
-    HIP_TRY(mr_convert(s, ctx->digits.as<mgcd_pair_t>(), {0}, {po.ndig}, ctx->mr, outs));
+    const int crc = mr_convert(s, ctx->digits.as<mgcd_pair_t>(), {0}, {po.ndig}, ctx->mr, outs);
+    if (crc != MGCD_OK) return crc;
     *ng = mag.size();
     if (mag.size() > cap) return MGCD_ERR_CAPACITY;
     if (!mag.empty()) std::memcpy(g, mag.data(), mag.size() * 4);
@@ -781,7 +785,8 @@ int mgcd_mixed_radix_to_limbs(mgcd_ctx* ctx, const mgcd_pair_t* digits, size_t k
   HIP_TRY(hipMemcpyAsync(ctx->digits.p, digits, k * 8, hipMemcpyHostToDevice, ctx->stream));
   std::vector<uint32_t> mag;
   std::vector<std::vector<uint32_t>*> outs{&mag};
-  HIP_TRY(mr_convert(ctx->stream, ctx->digits.as<mgcd_pair_t>(), {0}, {(uint32_t)k}, ctx->mr, outs));
+  const int crc = mr_convert(ctx->stream, ctx->digits.as<mgcd_pair_t>(), {0}, {(uint32_t)k}, ctx->mr, outs);
+  if (crc != MGCD_OK) return crc;
   *ng = mag.size();
   if (mag.size() > cap) return MGCD_ERR_CAPACITY;
   if (!mag.empty()) std::memcpy(g, mag.data(), mag.size() * 4);
@@ -912,7 +917,7 @@ KernelArgs batch_args(const mgcd_batch* b, const BatchLaunch& l) {
   ka.trace = b->trace.as<mgcd_pair_t>();
   ka.res = b->res.as<uint32_t>();
   ka.out = b->out.as<PairOut>();
-  ka.spin_limit = kSpinLimit;
+  ka.spin_limit = ctx->spin_limit;
   ka.relay_min_g = ctx->relay_min_g;
   ka.nqueues = l.ls.nqueues;
   std::memcpy(ka.qoff, l.ls.qoff, sizeof ka.qoff);
@@ -1238,7 +1243,8 @@ int mgcd_batch_run(mgcd_batch* b) {
         ks.push_back(o.ndig);
         outs.push_back(&b->host_result[i]);
       }
-      HIP_TRY(mr_convert(s, b->digits.as<mgcd_pair_t>(), doff, ks, b->mr, outs));
+      const int crc = mr_convert(s, b->digits.as<mgcd_pair_t>(), doff, ks, b->mr, outs);
+      if (crc != MGCD_OK) return crc;
     }
 
     std::atomic<size_t> next{0};
@@ -1639,7 +1645,7 @@ int server_launch(mgcd_server* s, ServerSlot& sl) {
   ka.trace = sl.trace.as<mgcd_pair_t>();
   ka.res = sl.res.as<uint32_t>();
   ka.out = sl.out.as<PairOut>();
-  ka.spin_limit = kSpinLimit;
+  ka.spin_limit = ctx->spin_limit;
   ka.relay_min_g = ctx->relay_min_g;
   ka.nqueues = sl.ls.nqueues;
   std::memcpy(ka.qoff, sl.ls.qoff, sizeof ka.qoff);

@@ -49,6 +49,7 @@ OPT_AUTOTUNE = 5
 OPT_BATCH_GRAPH = 6
 OPT_RELAY_PARTS = 7
 OPT_DEVICE_CONVERT = 8
+OPT_SPIN_LIMIT = 9
 
 
 class Pair(ctypes.Structure):
