@@ -685,7 +685,9 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         for (int b = 0; b < B; ++b) {
           // Branch-free: every candidate is computed, then selected.
           const double tv = t[b][k];
-          const double d = tv - load_after(bbp, b, b == 0 ? tv : P[b - 1]);
+          // The pivot value is read one product ahead of its use, so the
+          // LDS latency overlaps the previous multiplication.
+          const double d = tv - load_after(bbp, b, b >= 2 ? P[b - 2] : tv);
           bool ok = __builtin_fabs(tv) < TFIN;
           if constexpr (decltype(full)::value) ok = ok & (d != 0.0) & (__builtin_fabs(d) != M[k].m);
           okv[b] = ok;
@@ -702,9 +704,10 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
 #pragma unroll
       for (int b = B - 1; b >= 0; --b) {
         const double tv = t[b][k];
+        const double ppv = load_after(ppp, b, r);  // read two multiplications ahead of its use
         const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
         if (b > 0) r = A::mulmod(r, Y[b], M[k]);
-        const double tn = tsigned(A::mulmod(load_after(ppp, b, inv), inv, M[k]), M[k]);
+        const double tn = tsigned(A::mulmod(ppv, inv, M[k]), M[k]);
         // Every value a non-invertible t can take next (0, TINF and the
         // sentinels TZERO, TRET) has a zero low word: the selects work on
         // the high words only.
