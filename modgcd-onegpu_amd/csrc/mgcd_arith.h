@@ -72,6 +72,8 @@ __device__ __forceinline__ float near_quo_f(float xf, float yf) {
 
 // Moduli up to 32 bits: products need the two-product low part.
 struct Arith32 {
+  // mulmod is exact for operands below 2^33 in magnitude, not only reduced ones.
+  static constexpr bool kWideOperands = true;
   static __device__ __forceinline__ double mulmod(double a, double b, ModD M) {
     const double h = a * b;
     const double l = __builtin_fma(a, b, -h);
@@ -107,6 +109,8 @@ struct Arith32 {
 // Moduli below 2^22: residues, products (< 2^44) and the whole extended
 // Euclid fit FP32/FP64 exactly, so the inverse runs in FP32 only.
 struct Arith22 {
+  // mulmod needs |a * b| < 2^53: operands below 2^31 in magnitude only.
+  static constexpr bool kWideOperands = false;
   static __device__ __forceinline__ double mulmod(double a, double b, ModD M) {
     const double h = a * b;  // exact: |a*b| < 2^46
     const double q = __builtin_rint(h * M.minv);

@@ -61,6 +61,9 @@ constexpr double TINF = 2147483648.0;    // vq == 0: the reference's MOD_INFINIT
 constexpr double TZERO = -2147483648.0;  // uq == vq == 0
 constexpr double TRET = 4294967296.0;    // retired modulus
 constexpr double TFIN = 2147483648.0;    // |t| < TFIN: finite
+// Within one round only: a finite t equal to its pair's pivot (t - b == 0
+// mod m), whose next state is TINF.  Same high word as TINF, low word 1.
+constexpr double TPOLE = 2147483648.0 + 0x1p-21;
 
 // Keys are doubles holding exact integers, compared with v_min_f64:
 //   elimination  |t| * 2^20 + 2 (IDX_MAX - idx) + [t < 0]
@@ -533,8 +536,11 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       M[k].m = args.mod_d[base + j];
       M[k].minv = args.mod_inv[base + j];
     } else {
-      M[k].m = 3.0;  // no modulus here: any modulus keeps the lane harmless
-      M[k].minv = 1.0 / 3.0;
+      // No modulus here: the lane's values are all sentinels, which enter
+      // the elimination's products as they are, so it takes a table
+      // modulus (a small one would make such a product 0 often).
+      M[k].m = args.mod_d[0];
+      M[k].minv = args.mod_inv[0];
     }
     ck[k] = 0.5 + 2.0 * (double)(IDX_MAX - (base + j));
   }
@@ -643,8 +649,11 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         }
       }
       if (lane < (uint32_t)B && !ps.live) {
-        S.bb[lane] = 0.0;
-        S.pp[lane] = 1.0;
+        // A pair that is done (or absent) keeps being updated with t' =
+        // 0 / (t - 1): its finite t settle at 0, whose y = -1 is never 0
+        // (mod m), so it sends no wave through the full forward pass.
+        S.bb[lane] = 1.0;
+        S.pp[lane] = 0.0;
         S.pidx[lane] = NONE_IDX;
       }
       const bool any = __ballot(ps.live != 0) != 0;
@@ -666,41 +675,49 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       }
     }
     // t' = p' / (t - b) for every pair (:379-385): prefix products of the
-    // values to invert (1 where there is nothing to invert), one inverse,
-    // then back down (Montgomery's batch inversion).
+    // values to invert, one inverse, then back down (Montgomery's batch
+    // inversion).
     wkey = KNONE;
     LdsPtr<double> bbp = lds_ptr(S.bb), ppp = lds_ptr(S.pp);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (k >= rows) break;
-      // Forward: the value to invert is y = t - b where that is invertible
-      // (a finite t with t != b mod m), else 1; prefix products of the y.
-      // t == b (mod m) is rare (about one value in m), so the first pass
-      // only tests that t is finite: a y == 0 (mod m) then makes the product
-      // 0 (mod m), and the wave redoes the pass with the full test.
+      // Forward: prefix products of the values to invert, y = t - b.  Where
+      // mulmod takes operands up to 2^33, every lane's y enters as it is,
+      // also where t is a sentinel (an integer below 2^33 like the others;
+      // that lane's result is replaced below); else y = 1 there.  A y == 0
+      // (mod m) makes the product 0 (mod m); that is rare (about one value
+      // in m), so only then does the wave redo the pass with y = 1 wherever
+      // t is a sentinel or t == b (mod m), and marks the latter TPOLE.
       double P[B], Y[B], key[B];
-      bool okv[B];
       auto forward = [&](auto full) {
 #pragma unroll
         for (int b = 0; b < B; ++b) {
-          // Branch-free: every candidate is computed, then selected.
           const double tv = t[b][k];
           // The pivot value is read one product ahead of its use, so the
           // LDS latency overlaps the previous multiplication.
           const double d = tv - load_after(bbp, b, b >= 2 ? P[b - 2] : tv);
-          bool ok = __builtin_fabs(tv) < TFIN;
-          if constexpr (decltype(full)::value) ok = ok & (d != 0.0) & (__builtin_fabs(d) != M[k].m);
-          okv[b] = ok;
-          Y[b] = mk64(ok ? lo32(d) : 0u, ok ? hi32(d) : hi32(1.0));
-          P[b] = b == 0 ? Y[b] : A::mulmod(P[b - 1], Y[b], M[k]);
+          double y = d;
+          if constexpr (decltype(full)::value) {
+            const bool fin = __builtin_fabs(tv) < TFIN;
+            const bool pole = fin & ((d == 0.0) | (__builtin_fabs(d) == M[k].m));
+            if (pole) t[b][k] = TPOLE;
+            y = fin & !pole ? d : 1.0;
+          } else if constexpr (!A::kWideOperands) {
+            y = __builtin_fabs(tv) < TFIN ? d : 1.0;
+          }
+          Y[b] = y;
+          P[b] = b == 0 ? y : A::mulmod(P[b - 1], y, M[k]);
         }
+        if constexpr (B == 1) P[0] = reduce(P[0], M[k]);
       };
       forward(std::false_type{});
-      // P[B-1] is a reduced product (|P| < m) or, for B == 1, y itself (|y| <= m + 1).
+      // |P[B-1]| < m/2 + 2^15: a product, or for B == 1 the reduced y.
       if (__ballot((P[B - 1] == 0.0) | (__builtin_fabs(P[B - 1]) == M[k].m))) forward(std::true_type{});
       double r = A::modinv(canon(P[B - 1], M[k]), M[k]);
-      // Backward: t' = p' / y where y was invertible; elsewhere t - b == 0
-      // gives infinity, infinity gives 0, retired and zero states stay.
+      // Backward: t' = p' / y where t is finite.  The sentinels' next states
+      // all have a zero low word: TINF gives 0, TPOLE gives TINF, TZERO and
+      // TRET stay, so the selects work on the high words only.
 #pragma unroll
       for (int b = B - 1; b >= 0; --b) {
         const double tv = t[b][k];
@@ -708,12 +725,9 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
         if (b > 0) r = A::mulmod(r, Y[b], M[k]);
         const double tn = tsigned(A::mulmod(ppv, inv, M[k]), M[k]);
-        // Every value a non-invertible t can take next (0, TINF and the
-        // sentinels TZERO, TRET) has a zero low word: the selects work on
-        // the high words only.
         const bool fin = __builtin_fabs(tv) < TFIN;
-        const uint32_t alt_hi = tv == TINF ? 0u : (fin ? hi32(TINF) : hi32(tv));
-        const double tv2 = mk64(okv[b] ? lo32(tn) : 0u, okv[b] ? hi32(tn) : alt_hi);
+        const uint32_t alt_hi = tv == TINF ? 0u : hi32(tv);
+        const double tv2 = mk64(fin ? lo32(tn) : 0u, fin ? hi32(tn) : alt_hi);
         t[b][k] = tv2;
         key[b] = elim_key(tv2, ck[k]);
       }
@@ -745,8 +759,11 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         }
       }
       if (lane < (uint32_t)B && !ps.live) {
-        S.bb[lane] = 0.0;
-        S.pp[lane] = 1.0;
+        // A pair that is done (or absent) keeps being updated with t' =
+        // 0 / (t - 1): its finite t settle at 0, whose y = -1 is never 0
+        // (mod m), so it sends no wave through the full forward pass.
+        S.bb[lane] = 1.0;
+        S.pp[lane] = 0.0;
         S.pidx[lane] = NONE_IDX;
       }
       const bool any = __ballot(ps.live != 0) != 0;
