@@ -102,6 +102,16 @@ __device__ __forceinline__ double tsigned(double r, ModD M) {
   return __builtin_fma(-q, M.m, r);
 }
 
+// s_setprio with a level known once loops are unrolled (below 0: 0).
+__device__ __forceinline__ void set_prio(int p) {
+  switch (p) {
+    case 3: __builtin_amdgcn_s_setprio(3); break;
+    case 2: __builtin_amdgcn_s_setprio(2); break;
+    case 1: __builtin_amdgcn_s_setprio(1); break;
+    default: __builtin_amdgcn_s_setprio(0); break;
+  }
+}
+
 // __ffs(abs(x)) as at getGcdKernel.cu:391 (abs(INT32_MIN) stays INT32_MIN).
 __device__ __forceinline__ int ffs_abs(int32_t x) {
   const uint32_t a = x < 0 ? 0u - (uint32_t)x : (uint32_t)x;
@@ -613,6 +623,11 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
     __syncthreads();
     double x = KNONE;
     if (wave == 0) {
+      // The reduction, the exchange and the bookkeeping after it are the
+      // serial part of a round: in group tilings wave 0 runs them ahead of
+      // the waves of any other workgroup on the CU.  (One-pair tilings,
+      // whose exchange dominates the round, measured slower with it.)
+      if constexpr (B > 1) set_prio(3);
       if (lane < (uint32_t)B) {
         x = S.red[lane][0];
 #pragma unroll
@@ -658,6 +673,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       }
       const bool any = __ballot(ps.live != 0) != 0;
       if (lane == 0) S.flag = any;
+      if constexpr (B > 1) set_prio(0);
     }
     __syncthreads();
   }
@@ -711,10 +727,16 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         }
         if constexpr (B == 1) P[0] = reduce(P[0], M[k]);
       };
+      // Issue priority falls as a wave moves through the round's phases
+      // (forward, backward; row by row): a wave that has fallen behind the
+      // other wave on its SIMD gets the issue slots until it catches up, so
+      // the two finish together instead of one running alone at the end.
+      set_prio(3 - 2 * k);
       forward(std::false_type{});
       // |P[B-1]| < m/2 + 2^15: a product, or for B == 1 the reduced y.
       if (__ballot((P[B - 1] == 0.0) | (__builtin_fabs(P[B - 1]) == M[k].m))) forward(std::true_type{});
       double r = A::modinv(canon(P[B - 1], M[k]), M[k]);
+      set_prio(2 - 2 * k);
       // Backward: t' = p' / y where t is finite.  The sentinels' next states
       // all have a zero low word: TINF gives 0, TPOLE gives TINF, TZERO and
       // TRET stay, so the selects work on the high words only.
@@ -733,6 +755,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       }
       wkey = __builtin_fmin(wkey, wave_min_scatter<B>(key));
     }
+    __builtin_amdgcn_s_setprio(0);
     const double kr = reduce_keys(wkey);
     if (wave == 0) {
       if (lane < npairs && ps.live) {
@@ -768,6 +791,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       }
       const bool any = __ballot(ps.live != 0) != 0;
       if (lane == 0) S.flag = any;
+      if constexpr (B > 1) set_prio(0);
     }
     __syncthreads();
   }

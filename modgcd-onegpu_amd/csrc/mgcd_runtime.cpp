@@ -1015,8 +1015,14 @@ int plan_batch(mgcd_batch* b, int forced_variant, const std::vector<int>& class_
   return MGCD_OK;
 }
 
-// Key of a tuned tiling: width L and size class.
-std::string tune_key(int L, int cls) { return std::to_string(L) + ":" + std::to_string(cls); }
+// Key of a tuned tiling: width L, size class and the class's population
+// (power of two, up to 2^12): how many groups a class fills the device
+// with decides between tilings as much as the pairs' size does.
+std::string tune_key(int L, int cls, size_t count) {
+  int p = 0;
+  while (p < 12 && (size_t(2) << p) <= count) ++p;
+  return std::to_string(L) + ":" + std::to_string(cls) + ":" + std::to_string(p);
+}
 
 void load_tune_cache(mgcd_ctx* ctx) {
   const char* path = std::getenv("MGCD_TUNE_CACHE");
@@ -1107,14 +1113,16 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
     for (size_t i = 0; i < npairs; ++i)
       if (b->kidx[i] >= 0) byclass[class_of(b->pd[b->kidx[i]].nq)].push_back(i);
     bool tuned_any = false;
-    for (int c = 0; c < kNumClasses && rc == MGCD_OK; ++c) {
-      if (byclass[c].empty()) continue;
-      const auto it = ctx->tuned.find(tune_key(ctx->L, c));
+    auto pick = [&](int c) {
+      const std::string key = tune_key(ctx->L, c, byclass[c].size());
+      const auto it = ctx->tuned.find(key);
       if (it != ctx->tuned.end()) {
         class_variant[c] = it->second;
       } else if (ctx->autotune) {
-        // A sample of the class, evenly spaced.
-        const size_t want = std::min<size_t>(byclass[c].size(), 512);
+        // The class itself, or an evenly spaced sample of 4096 of its pairs:
+        // a smaller sample fills the device with fewer groups than the class
+        // does, which favours the small-group tilings.
+        const size_t want = std::min<size_t>(byclass[c].size(), 4096);
         std::vector<uint64_t> so, to;
         std::vector<uint32_t> sn, tn;
         for (size_t s = 0; s < want; ++s) {
@@ -1128,12 +1136,28 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
         rc = mgcd_autotune(ctx, limbs, so.data(), sn.data(), to.data(), tn.data(), want, &T, &K, &Bq);
         if (rc == MGCD_OK) {
           class_variant[c] = find_kernel_variant(T, K, Bq, ctx->wide);
-          ctx->tuned[tune_key(ctx->L, c)] = class_variant[c];
+          ctx->tuned[key] = class_variant[c];
           tuned_any = true;
         }
       } else {
         class_variant[c] = default_variant(ctx, c);
       }
+    };
+    // Elimination work of each class, about N_Q^2 per pair.  A class with
+    // under a tenth of the largest one's work takes that class's tiling: it
+    // then shares its launch, filling the workgroups that the large items
+    // leave idle, instead of running in a launch of its own.
+    std::vector<double> work(kNumClasses, 0.0);
+    int cmax = -1;
+    for (int c = 0; c < kNumClasses; ++c) {
+      for (size_t i : byclass[c]) work[c] += (double)b->pd[b->kidx[i]].nq * (double)b->pd[b->kidx[i]].nq;
+      if (!byclass[c].empty() && (cmax < 0 || work[c] > work[cmax])) cmax = c;
+    }
+    if (cmax >= 0) pick(cmax);
+    for (int c = 0; c < kNumClasses && rc == MGCD_OK; ++c) {
+      if (byclass[c].empty() || c == cmax) continue;
+      if (work[c] < 0.1 * work[cmax]) class_variant[c] = class_variant[cmax];
+      else pick(c);
     }
     if (tuned_any) save_tune_cache(ctx);
   }

@@ -15,22 +15,50 @@ def _mix(wl, n, seed):
     return wl.mixed(n=n, seed=seed, lo=1000, hi=200000)
 
 
+def _check(b, pairs):
+    for i, (u, v) in enumerate(pairs):
+        assert b.result(i) == math.gcd(u, v), i
+
+
 def test_size_classes_get_their_own_tilings(ctx_for, mgcd, wl):
     ctx = ctx_for(32)
-    pairs = _mix(wl, 96, 31)
+    # Two classes of comparable elimination work (about N_Q^2 per pair):
+    # small pairs (N_Q <= 1024) run with their own tiling and launch.
+    small = wl.headline(256, seed=31, nbits=5000, gbits=500)
+    mid = wl.headline(8, seed=35, nbits=30000, gbits=3000)
+    assert mgcd.num_moduli_needed(5000, 32) <= 1024 < mgcd.num_moduli_needed(30000, 32) <= 4096
+    pairs = small + mid
     b = mgcd.Batch.from_pairs(ctx, pairs)
     b.run()
     launches = b.launches()
-    # Small pairs (N_Q <= 1024) run with their own tiling; adjacent classes
-    # with equal tilings share a launch.
-    assert len(launches) >= 2
-    assert all(x["tiling"] != y["tiling"] for x, y in zip(launches, launches[1:]))
+    assert len(launches) == 2
+    assert launches[0]["tiling"] != launches[1]["tiling"]
     assert launches[0]["nq"][1] <= 1024 < launches[1]["nq"][0]
-    assert sum(x["pairs"] for x in launches) == sum(1 for u, v in pairs if u and v)
+    assert [x["pairs"] for x in launches] == [256, 8]
     for x in launches:
         assert x["device_s"] > 0
-    for i, (u, v) in enumerate(pairs):
-        assert b.result(i) == math.gcd(u, v), i
+    _check(b, pairs)
+    b.close()
+
+
+def test_small_class_shares_the_dominant_launch(ctx_for, mgcd, wl):
+    ctx = ctx_for(32)
+    # Under a tenth of the largest class's work: the small pairs take the
+    # large ones' tiling and run in their launch.
+    pairs = wl.headline(8, seed=36, nbits=5000, gbits=500) + wl.headline(8, seed=37, nbits=30000, gbits=3000)
+    b = mgcd.Batch.from_pairs(ctx, pairs)
+    b.run()
+    launches = b.launches()
+    assert len(launches) == 1 and launches[0]["pairs"] == 16
+    assert launches[0]["nq"][0] <= 1024 < launches[0]["nq"][1]
+    _check(b, pairs)
+    b.close()
+    # Any mix across all four classes stays exact.
+    pairs = _mix(wl, 96, 31)
+    b = mgcd.Batch.from_pairs(ctx, pairs)
+    b.run()
+    assert sum(x["pairs"] for x in b.launches()) == sum(1 for u, v in pairs if u and v)
+    _check(b, pairs)
     b.close()
 
 
@@ -78,7 +106,7 @@ def test_autotuned_batch_and_cache_file(mgcd, wl, tmp_path):
         b.run()
         assert b.results() == [math.gcd(u, v) for u, v in pairs]
         lines = cache.read_text().split()
-        assert len(lines) >= 4 and len(lines) % 4 == 0  # "L:class T K B" per tuned class
+        assert len(lines) >= 4 and len(lines) % 4 == 0  # "L:class:population T K B" per tuned class
         b.close()
         ctx.close()
         # A new context reads the tuned tilings back instead of measuring.
