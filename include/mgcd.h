@@ -92,10 +92,12 @@ enum {
   MGCD_OPT_SINGLE_WORKGROUP = 3,
   MGCD_OPT_SINGLE_SLOTS = 4,
   /* 1: batches created with tiling (0, 0, 0) measure the candidate tilings
-   * on a sample of each size class (mgcd_autotune) unless a tuned tiling for
-   * that class is cached in the context or in the file named by
-   * $MGCD_TUNE_CACHE, which receives the new results.  0 (default): cached
-   * or built-in tilings. */
+   * on each size class (up to 4096 of its pairs; mgcd_autotune) unless a
+   * tuned tiling for that class and population is cached in the context or
+   * in the file named by $MGCD_TUNE_CACHE, which receives the new results.
+   * 0 (default): cached or built-in tilings.  Either way a class with under
+   * a tenth of the largest class's work (sum of N_Q^2) runs with that
+   * class's tiling, in its launch. */
   MGCD_OPT_AUTOTUNE = 5,
   /* 1: mgcd_batch_run captures the batch's launch sequence into a hipGraph
    * on its first run and replays the graph afterwards.  0 (default):
