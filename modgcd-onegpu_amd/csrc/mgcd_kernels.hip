@@ -245,6 +245,7 @@ struct Shared {
   uint32_t ntrace[B];
   double sel[B];              // recovery: the owner's (index, value) key
   uint32_t retire[T * K];     // bit b: this part's modulus is pair b's pivot
+  uint32_t retw[K][NW];       // per slot row and wave: OR of its retire words
   uint32_t item;
   // Staging for (a) and (c), and during (b) the group's moduli (when N_Q
   // fits), so that a pivot's modulus is an LDS read instead of a global one
@@ -641,10 +642,14 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
 
 #pragma unroll
   for (int k = 0; k < K; ++k) S.retire[k * T + tid] = 0;
+  if (tid < (uint32_t)(K * NW)) (&S.retw[0][0])[tid] = 0;
   // Marks pair b's pivot modulus for retirement if it lies in this part.
   auto mark_retire = [&](uint32_t b, uint32_t idx) {
     const uint32_t off = idx - base;
-    if (off < cnt) atomicOr(&S.retire[off], 1u << b);
+    if (off < cnt) {
+      atomicOr(&S.retire[off], 1u << b);
+      atomicOr(&S.retw[off / T][(off % T) >> 6], 1u << b);
+    }
   };
 
   // First pivot: the loop body runs once even for an infinite one (do-while, :374-402).
@@ -679,15 +684,28 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   }
 
   while (S.flag) {
-    // Retire each pair's pivot modulus (:376-377); few waves hold one.
+    // Retire each pair's pivot modulus (:376-377).  Few waves hold one; a
+    // wave that does visits only the pairs named in its summary word
+    // (uniform), with one select per pair.
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const uint32_t rw = S.retire[k * T + tid];
-      if (__ballot(rw != 0)) {
-        if (rw) S.retire[k * T + tid] = 0;
+      uint32_t sw = __builtin_amdgcn_readfirstlane(S.retw[k][wave]);
+      if (sw) {
+        const uint32_t rw = S.retire[k * T + tid];
+        S.retire[k * T + tid] = 0;
+        if (lane == 0) S.retw[k][wave] = 0;
+        while (sw) {
+          const uint32_t b = (uint32_t)__builtin_ctz(sw);
+          sw &= sw - 1;
+          // A uniform branch per candidate pair (the empty asm keeps the
+          // compiler from turning the B branches into B selects).
 #pragma unroll
-        for (int b = 0; b < B; ++b)
-          if ((rw >> b) & 1) t[b][k] = TRET;
+          for (int c = 0; c < B; ++c)
+            if ((uint32_t)c == b) {
+              asm volatile("" ::: "memory");
+              if ((rw >> c) & 1) t[c][k] = TRET;
+            }
+        }
       }
     }
     // t' = p' / (t - b) for every pair (:379-385): prefix products of the
