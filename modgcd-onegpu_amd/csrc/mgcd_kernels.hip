@@ -568,11 +568,14 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   // for pair l >> SH.
   double t[B][K];
   double wkey = KNONE;
+  // Bit k: some lane of this wave holds TINF in slot row k (uniform).
+  uint32_t infw = 0;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const uint32_t j = (uint32_t)k * T + tid;
     const bool here = j < cnt;
     double P[B], key[B];
+    bool anyinf = false;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       double v = 0.0;
@@ -593,9 +596,11 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       if (b > 0) r = A::mulmod(r, v != 0.0 ? v : 1.0, M[k]);
       double tv = v != 0.0 ? tsigned(A::mulmod(u, inv, M[k]), M[k]) : (u != 0.0 ? TINF : TZERO);
       if (!here) tv = TRET;
+      anyinf |= tv == TINF;
       t[b][k] = tv;
       key[b] = elim_key(tv, ck[k]);
     }
+    if (__ballot(anyinf)) infw |= 1u << k;
     wkey = __builtin_fmin(wkey, wave_min_scatter<B>(key));
   }
 
@@ -724,6 +729,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       // in m), so only then does the wave redo the pass with y = 1 wherever
       // t is a sentinel or t == b (mod m), and marks the latter TPOLE.
       double P[B], Y[B], key[B];
+      bool pole_any = false;
       auto forward = [&](auto full) {
 #pragma unroll
         for (int b = 0; b < B; ++b) {
@@ -736,6 +742,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
             const bool fin = __builtin_fabs(tv) < TFIN;
             const bool pole = fin & ((d == 0.0) | (__builtin_fabs(d) == M[k].m));
             if (pole) t[b][k] = TPOLE;
+            pole_any |= pole;
             y = fin & !pole ? d : 1.0;
           } else if constexpr (!A::kWideOperands) {
             y = __builtin_fabs(tv) < TFIN ? d : 1.0;
@@ -752,25 +759,41 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       set_prio(3 - 2 * k);
       forward(std::false_type{});
       // |P[B-1]| < m/2 + 2^15: a product, or for B == 1 the reduced y.
-      if (__ballot((P[B - 1] == 0.0) | (__builtin_fabs(P[B - 1]) == M[k].m))) forward(std::true_type{});
+      const bool redo = __ballot((P[B - 1] == 0.0) | (__builtin_fabs(P[B - 1]) == M[k].m)) != 0;
+      if (redo) forward(std::true_type{});
+      const bool poled = redo && __ballot(pole_any) != 0;
       double r = A::modinv(canon(P[B - 1], M[k]), M[k]);
       set_prio(2 - 2 * k);
-      // Backward: t' = p' / y where t is finite.  The sentinels' next states
-      // all have a zero low word: TINF gives 0, TPOLE gives TINF, TZERO and
-      // TRET stay, so the selects work on the high words only.
+      // Backward: t' = p' / y where t is finite.  A sentinel stays, except
+      // that TINF gives 0 and TPOLE gives TINF; all of them have a zero low
+      // word, so those two take one high-word select, in the rare rounds
+      // (uniform per wave and row) where the row holds one of them.
+      auto backward = [&](auto special) {
 #pragma unroll
-      for (int b = B - 1; b >= 0; --b) {
-        const double tv = t[b][k];
-        const double ppv = load_after(ppp, b, r);  // read two multiplications ahead of its use
-        const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
-        if (b > 0) r = A::mulmod(r, Y[b], M[k]);
-        const double tn = tsigned(A::mulmod(ppv, inv, M[k]), M[k]);
-        const bool fin = __builtin_fabs(tv) < TFIN;
-        const uint32_t alt_hi = tv == TINF ? 0u : hi32(tv);
-        const double tv2 = mk64(fin ? lo32(tn) : 0u, fin ? hi32(tn) : alt_hi);
-        t[b][k] = tv2;
-        key[b] = elim_key(tv2, ck[k]);
-      }
+        for (int b = B - 1; b >= 0; --b) {
+          const double tv = t[b][k];
+          const double ppv = load_after(ppp, b, r);  // read two multiplications ahead of its use
+          const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
+          if (b > 0) r = A::mulmod(r, Y[b], M[k]);
+          const double tn = tsigned(A::mulmod(ppv, inv, M[k]), M[k]);
+          const bool fin = __builtin_fabs(tv) < TFIN;
+          double tv2;
+          if constexpr (decltype(special)::value) {
+            const uint32_t alt_hi = tv == TINF ? 0u : hi32(tv);
+            tv2 = mk64(fin ? lo32(tn) : 0u, fin ? hi32(tn) : alt_hi);
+          } else {
+            tv2 = fin ? tn : tv;
+          }
+          t[b][k] = tv2;
+          key[b] = elim_key(tv2, ck[k]);
+        }
+      };
+      // (One-pair tilings, whose round is mostly the exchange, measured
+      // slower with the two versions: they keep the general one.)
+      if (B == 1 || (((infw >> k) & 1) | poled)) backward(std::true_type{});
+      else backward(std::false_type{});
+      // TINF lanes become 0 this round; the poles' TINF stay one round.
+      infw = (infw & ~(1u << k)) | ((uint32_t)poled << k);
       wkey = __builtin_fmin(wkey, wave_min_scatter<B>(key));
     }
     __builtin_amdgcn_s_setprio(0);
