@@ -98,10 +98,16 @@ struct Arith32 {
     float f0 = (float)M.m, f1 = (float)y;
     for (;;) {
       eea_step(r0, f0, s0, r1, f1, s1);
-      if (!(__builtin_fabs(r0) > 1.0)) { s = (r0 != 0.0) ? r0 * s0 : r1 * s1; break; }
+      if (!(__builtin_fabs(r0) > 1.0)) break;
       eea_step(r1, f1, s1, r0, f0, s0);
-      if (!(__builtin_fabs(r1) > 1.0)) { s = (r1 != 0.0) ? r1 * s1 : r0 * s0; break; }
+      if (!(__builtin_fabs(r1) > 1.0)) break;
     }
+    // The remainder computed last is the one with |r| <= 1 (r0 if the loop
+    // left after its first step); the other holds the one before it.
+    const bool last0 = !(__builtin_fabs(r0) > 1.0);
+    const double rl = last0 ? r0 : r1, sl = last0 ? s0 : s1;
+    const double rp = last0 ? r1 : r0, sp = last0 ? s1 : s0;
+    s = rl != 0.0 ? rl * sl : rp * sp;
     return canon(reduce(s, M), M);
   }
 };
