@@ -131,10 +131,14 @@ struct Arith22 {
     float r0 = (float)M.m, r1 = (float)y, s0 = 0.0f, s1 = 1.0f, s;
     for (;;) {
       eea_step(r0, s0, r1, s1);
-      if (!(__builtin_fabsf(r0) > 1.0f)) { s = (r0 != 0.0f) ? r0 * s0 : r1 * s1; break; }
+      if (!(__builtin_fabsf(r0) > 1.0f)) break;
       eea_step(r1, s1, r0, s0);
-      if (!(__builtin_fabsf(r1) > 1.0f)) { s = (r1 != 0.0f) ? r1 * s1 : r0 * s0; break; }
+      if (!(__builtin_fabsf(r1) > 1.0f)) break;
     }
+    const bool last0 = !(__builtin_fabsf(r0) > 1.0f);  // as in Arith32::modinv
+    const float rl = last0 ? r0 : r1, sl = last0 ? s0 : s1;
+    const float rp = last0 ? r1 : r0, sp = last0 ? s1 : s0;
+    s = rl != 0.0f ? rl * sl : rp * sp;
     return canon(reduce((double)s, M), M);
   }
 };

@@ -143,8 +143,10 @@ __device__ __forceinline__ double mk64(uint32_t lo, uint32_t hi) {
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp64(double x) {
-  return mk64((uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo32(x), CTRL, 0xF, 0xF, false),
-              (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi32(x), CTRL, 0xF, 0xF, false));
+  // Every control used here is a permutation within a row, so every lane
+  // has a source and the moves need no old value.
+  return mk64((uint32_t)__builtin_amdgcn_mov_dpp((int)lo32(x), CTRL, 0xF, 0xF, false),
+              (uint32_t)__builtin_amdgcn_mov_dpp((int)hi32(x), CTRL, 0xF, 0xF, false));
 }
 // v_permlane32_swap: the upper 32 lanes of a trade places with the lower
 // 32 lanes of b.  v_permlane16_swap: within each half, the odd row (16
@@ -170,6 +172,14 @@ template <int S>
 constexpr int stage_dpp() {
   return S == 2 ? 0x140 : S == 3 ? 0x141 : S == 4 ? 0x4E : 0xB1;
 }
+// The lanes whose index has bit b set, as a wave mask (a constant: a
+// select on it needs no per-lane test).
+constexpr uint64_t lane_bit_mask(int b) {
+  uint64_t m = 0;
+  for (int l = 0; l < 64; ++l)
+    if ((l >> b) & 1) m |= 1ull << l;
+  return m;
+}
 // Halving step: a lane whose bit is 0 keeps the minimum of the first
 // halves (a) of itself and its partner, a lane whose bit is 1 of the
 // second halves (b); the result is left in a.
@@ -179,7 +189,7 @@ __device__ __forceinline__ void half_step(double& a, double& b) {
     swap64<S>(a, b);
     a = __builtin_fmin(a, b);
   } else {
-    const bool hi = (__lane_id() >> (5 - S)) & 1;
+    const bool hi = __builtin_amdgcn_inverse_ballot_w64(lane_bit_mask(5 - S));
     const double send = hi ? a : b;
     const double keep = hi ? b : a;
     a = __builtin_fmin(keep, dpp64<stage_dpp<S>()>(send));
