@@ -763,9 +763,10 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         if constexpr (B == 1) P[0] = reduce(P[0], M[k]);
       };
       // Issue priority falls as a wave moves through the round's phases
-      // (forward, backward; row by row): a wave that has fallen behind the
-      // other wave on its SIMD gets the issue slots until it catches up, so
-      // the two finish together instead of one running alone at the end.
+      // (forward pass and first half of the backward pass, then its second
+      // half; row by row): a wave that has fallen behind the other wave on
+      // its SIMD gets the issue slots until it catches up, so the two finish
+      // together instead of one running alone at the end.
       set_prio(3 - 2 * k);
       forward(std::false_type{});
       // |P[B-1]| < m/2 + 2^15: a product, or for B == 1 the reduced y.
@@ -773,7 +774,6 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       if (redo) forward(std::true_type{});
       const bool poled = redo && __ballot(pole_any) != 0;
       double r = A::modinv(canon(P[B - 1], M[k]), M[k]);
-      set_prio(2 - 2 * k);
       // Backward: t' = p' / y where t is finite.  A sentinel stays, except
       // that TINF gives 0 and TPOLE gives TINF; all of them have a zero low
       // word, so those two take one high-word select, in the rare rounds
@@ -781,6 +781,9 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       auto backward = [&](auto special) {
 #pragma unroll
         for (int b = B - 1; b >= 0; --b) {
+          // The priority steps down halfway through the backward pass: the
+          // forward pass and the inverse are the shorter part of a row.
+          if (b == B / 2 - 1 || (B == 1 && b == 0)) set_prio(2 - 2 * k);
           const double tv = t[b][k];
           const double ppv = load_after(ppp, b, r);  // read two multiplications ahead of its use
           const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
