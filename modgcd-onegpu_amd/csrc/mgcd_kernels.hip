@@ -804,7 +804,11 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       // half; row by row): a wave that has fallen behind the other wave on
       // its SIMD gets the issue slots until it catches up, so the two finish
       // together instead of one running alone at the end.
-      set_prio(3 - 2 * k);
+      if constexpr (K == 2 && B >= 4) {
+        if (k == 0) set_prio(3);
+      } else {
+        set_prio(3 - 2 * k);
+      }
       forward(std::false_type{});
       // |P[B-1]| < m/2 + 2^15: a product, or for B == 1 the reduced y.
       const bool redo = __ballot((P[B - 1] == 0.0) | (__builtin_fabs(P[B - 1]) == M[k].m)) != 0;
@@ -820,7 +824,16 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         for (int b = B - 1; b >= 0; --b) {
           // The priority steps down halfway through the backward pass: the
           // forward pass and the inverse are the shorter part of a row.
-          if (b == B / 2 - 1 || (B == 1 && b == 0)) set_prio(2 - 2 * k);
+          // (Two rows: levels 3 | 2 from halfway through the first backward
+          // pass | 1 from the second | 0 for its last quarter, which keeps
+          // the final, equal-priority stretch short.)
+          if constexpr (K == 2 && B >= 4) {
+            if (k == 0 && b == B / 2 - 1) set_prio(2);
+            if (k == 1 && b == B - 1) set_prio(1);
+            if (k == 1 && b == B / 4 - 1) set_prio(0);
+          } else {
+            if (b == B / 2 - 1 || (B == 1 && b == 0)) set_prio(2 - 2 * k);
+          }
           const double tv = t[b][k];
           const double ppv = load_after(ppp, b, r);  // read two multiplications ahead of its use
           const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
