@@ -88,7 +88,8 @@ struct Arith32 {
     sx = __builtin_fma(-q, sy, sx);
     fx = (float)x;
   }
-  // y in [1, m) canonical; returns y^-1 mod m in [0, m).
+  // 0 < |y| < m, either sign (q(x, -y) = -q(x, y): the steps mirror those
+  // for |y|); returns y^-1 mod m in [0, m).
   // Invariant r_i == s_i * y (mod m); unrolled by two so the pair swaps
   // roles instead of moving registers; stops when |r| <= 1.  The last
   // nonzero remainder is +-1 (gcd(m, y) = 1): the inverse is its cofactor

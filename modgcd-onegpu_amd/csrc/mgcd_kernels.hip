@@ -810,11 +810,12 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         set_prio(3 - 2 * k);
       }
       forward(std::false_type{});
-      // |P[B-1]| < m/2 + 2^15: a product, or for B == 1 the reduced y.
+      // |P[B-1]| < m/2 + 2^15: a product, or for B == 1 the reduced y; the
+      // inverse takes it signed.
       const bool redo = __ballot((P[B - 1] == 0.0) | (__builtin_fabs(P[B - 1]) == M[k].m)) != 0;
       if (redo) forward(std::true_type{});
       const bool poled = redo && __ballot(pole_any) != 0;
-      double r = A::modinv(canon(P[B - 1], M[k]), M[k]);
+      double r = A::modinv(P[B - 1], M[k]);
       // Backward: t' = p' / y where t is finite.  A sentinel stays, except
       // that TINF gives 0 and TPOLE gives TINF; all of them have a zero low
       // word, so those two take one high-word select, in the rare rounds
