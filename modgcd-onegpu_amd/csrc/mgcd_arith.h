@@ -89,7 +89,8 @@ struct Arith32 {
     fx = (float)x;
   }
   // 0 < |y| < m, either sign (q(x, -y) = -q(x, y): the steps mirror those
-  // for |y|); returns y^-1 mod m in [0, m).
+  // for |y|); returns y^-1 mod m, signed reduced (|r| < m/2 + 1: every
+  // caller only multiplies with it).
   // Invariant r_i == s_i * y (mod m); unrolled by two so the pair swaps
   // roles instead of moving registers; stops when |r| <= 1.  The last
   // nonzero remainder is +-1 (gcd(m, y) = 1): the inverse is its cofactor
@@ -109,7 +110,7 @@ struct Arith32 {
     const double rl = last0 ? r0 : r1, sl = last0 ? s0 : s1;
     const double rp = last0 ? r1 : r0, sp = last0 ? s1 : s0;
     s = rl != 0.0 ? rl * sl : rp * sp;
-    return canon(reduce(s, M), M);
+    return reduce(s, M);
   }
 };
 
@@ -140,7 +141,7 @@ struct Arith22 {
     const float rl = last0 ? r0 : r1, sl = last0 ? s0 : s1;
     const float rp = last0 ? r1 : r0, sp = last0 ? s1 : s0;
     s = rl != 0.0f ? rl * sl : rp * sp;
-    return canon(reduce((double)s, M), M);
+    return reduce((double)s, M);
   }
 };
 
