@@ -630,7 +630,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       const double y = v != 0.0 ? v : 1.0;
       P[b] = b == 0 ? y : A::mulmod(P[b - 1], y, M[k]);
     }
-    double r = A::modinv(canon(P[B - 1], M[k]), M[k]);
+    double r = A::modinv(P[B - 1], M[k]);
 #pragma unroll
     for (int b = B - 1; b >= 0; --b) {
       double u = 0.0, v = 0.0;
@@ -810,9 +810,9 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         set_prio(3 - 2 * k);
       }
       forward(std::false_type{});
-      // |P[B-1]| < m/2 + 2^15: a product, or for B == 1 the reduced y; the
-      // inverse takes it signed.
-      const bool redo = __ballot((P[B - 1] == 0.0) | (__builtin_fabs(P[B - 1]) == M[k].m)) != 0;
+      // |P[B-1]| < m/2 + 2^15 (a product, or for B == 1 the reduced y), so it
+      // is 0 (mod m) only as 0; the inverse takes it signed.
+      const bool redo = __ballot(P[B - 1] == 0.0) != 0;
       if (redo) forward(std::true_type{});
       const bool poled = redo && __ballot(pole_any) != 0;
       double r = A::modinv(P[B - 1], M[k]);
@@ -1090,7 +1090,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
           const double y = ((own[k] >> j) & 1) ? us[j][k] : 1.0;
           P[j] = j == 0 ? y : A::mulmod(P[j - 1], y, M[k]);
         }
-        double r = A::modinv(canon(P[RB - 1], M[k]), M[k]);
+        double r = A::modinv(P[RB - 1], M[k]);
 #pragma unroll
         for (int j = RB - 1; j >= 0; --j) {
           const bool owns = (own[k] >> j) & 1;
