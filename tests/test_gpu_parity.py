@@ -32,6 +32,22 @@ def test_trace_small_L32(ctx_for, oracle, mgcd, wl, seed):
     assert R > 100 and k > 10
 
 
+@pytest.mark.parametrize("L", [32, 22])
+def test_trace_operands_divisible_by_table_moduli(ctx_for, oracle, mgcd, wl, L):
+    # Table moduli dividing v only (t starts at infinity there), both (0/0)
+    # and u only (t starts at 0, the first pivots): the sentinel states and
+    # their transitions against the oracle, pivot by pivot.
+    ctx = ctx_for(L)
+    mods = ctx.moduli()
+    rnd = random.Random(81 + L)
+    u, v = wl.single_small(3)
+    p = [int(mods[i]) for i in rnd.sample(range(200), 5)]
+    v = v * p[0] * p[1] * p[2] * p[3]
+    u = u * p[3] * p[4] * (rnd.getrandbits(160) | (1 << 159))
+    assert u > v
+    _check_trace(ctx, oracle, mgcd, u, v, L)
+
+
 @pytest.mark.parametrize("L", [27, 22])
 def test_trace_small_other_widths(ctx_for, oracle, mgcd, wl, L):
     u, v = wl.single_small(0)
@@ -173,6 +189,29 @@ def test_batch_tilings_agree(ctx_for, mgcd, wl, T, K, B):
     b.run()
     for i, (u, v) in enumerate(pairs):
         assert b.result(i) == math.gcd(u, v)
+
+
+@pytest.mark.parametrize("T, K, B", [(512, 2, 16), (256, 1, 16)])
+def test_batch_sentinel_states_match_single(ctx_for, mgcd, wl, T, K, B):
+    # Grouped pairs whose operands table moduli divide (t starting at
+    # infinity, 0/0 or 0 on those moduli), beside coprime pairs that finish
+    # at infinity early and stay in the group: each pair's iteration and
+    # digit counts and its gcd equal those of the pair run alone.
+    ctx = ctx_for(32)
+    mods = ctx.moduli()
+    rnd = random.Random(83)
+    pairs = []
+    for i, (u, v) in enumerate(wl.headline(12, seed=4, nbits=9000, gbits=2000)):
+        p = [int(mods[j]) for j in rnd.sample(range(600), 5)]
+        pairs.append((u * p[3] * p[4] * (rnd.getrandbits(160) | 1 << 159), v * p[0] * p[1] * p[2] * p[3]))
+    for i in range(4):
+        pairs.append((rnd.getrandbits(9000) | 1 << 8999 | 1, rnd.getrandbits(8990) | 1))
+    b = mgcd.Batch.from_pairs(ctx, pairs, workgroup=T, slots=K, group=B)
+    b.run()
+    for i, (u, v) in enumerate(pairs):
+        st, g, d, p = ctx.gcd_trace(max(u, v), min(u, v))
+        nq, R, k = b.pair_stats(i)
+        assert (R, k) == (len(p), len(d)) and b.result(i) == g == math.gcd(u, v), i
 
 
 def test_batch_group_traces_match_single(ctx_for, mgcd, wl):
