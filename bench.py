@@ -33,11 +33,11 @@ VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 # 4).  Units are the algorithm's own: active modulus-iterations, not the
 # lanes the kernel keeps busy after their modulus retired.
 OPS_PER_RESIDUE_STEP = 6        # one Horner step per limb per modulus
-OPS_PER_VALUE = 45              # per active modulus-iteration and pair: t' = p'/(t - b) with the
-                                # batch-inversion share (4 mulmod), toSigned, key, selects, reduce-scatter:
-                                # the (512, 2, 16) loop's common path, 157 forward + 518 backward VALU per
-                                # slot of 16 values, plus ~100 per iteration, over 32 values
-OPS_PER_EEA_STEP = 12           # one nearest-quotient extended-Euclid step (9 VALU + rcp)
+OPS_PER_VALUE = 40              # per active modulus-iteration and pair: t' = p'/(t - b) with the
+                                # batch-inversion share (4 mulmod), toSigned, key, select, reduce-scatter:
+                                # the (512, 2, 16) loop's common path, 1,187 VALU per iteration for its
+                                # 32 values (two slots of 16), plus ~100 per iteration, over 32 values
+OPS_PER_EEA_STEP = 11           # one nearest-quotient extended-Euclid step (7 VALU + rcp)
 EEA_STEPS = {32: 12.73, 27: 10.71, 22: 8.68}   # mean least-remainder Euclid steps, random y, m near 2^L
                                                # (CPU emulation, 2e6 each)
 OPS_PER_REPLAY = 19             # per surviving modulus and recorded pivot: 3 mulmod + subtract
