@@ -781,9 +781,9 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
 #pragma unroll
         for (int b = 0; b < B; ++b) {
           const double tv = t[b][k];
-          // The pivot value is read one product ahead of its use, so the
-          // LDS latency overlaps the previous multiplication.
-          const double d = tv - load_after(bbp, b, b >= 2 ? P[b - 2] : tv);
+          // The pivot value is read two products ahead of its use, so the
+          // LDS latency overlaps the multiplications before it.
+          const double d = tv - load_after(bbp, b, b >= 3 ? P[b - 3] : tv);
           double y = d;
           if constexpr (decltype(full)::value) {
             const bool fin = __builtin_fabs(tv) < TFIN;
