@@ -821,6 +821,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       // word, so those two take one high-word select, in the rare rounds
       // (uniform per wave and row) where the row holds one of them.
       auto backward = [&](auto special) {
+        double ppn = load_after(ppp, B - 1, r);
 #pragma unroll
         for (int b = B - 1; b >= 0; --b) {
           // The priority steps down halfway through the backward pass: the
@@ -836,7 +837,8 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
             if (b == B / 2 - 1 || (B == 1 && b == 0)) set_prio(2 - 2 * k);
           }
           const double tv = t[b][k];
-          const double ppv = load_after(ppp, b, r);  // read two multiplications ahead of its use
+          const double ppv = ppn;
+          if (b > 0) ppn = load_after(ppp, b - 1, r);  // read a step ahead of its use
           const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
           if (b > 0) r = A::mulmod(r, Y[b], M[k]);
           const double tn = tsigned(A::mulmod(ppv, inv, M[k]), M[k]);
