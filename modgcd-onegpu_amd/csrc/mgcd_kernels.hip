@@ -416,9 +416,10 @@ __device__ void residues(const uint32_t* (&x)[NOP], const uint32_t (&n)[NOP], do
         if ((uint32_t)q == o) tile[q * TS + jj] = i < n[q] ? (double)x[q][i] : 0.0;
     }
     __syncthreads();
-    // Unrolled by four with the group tilings' larger tiles (the limbs'
-    // LDS reads issue under the previous steps' arithmetic).
-    constexpr int RU = TL > TILE ? 4 : 1;
+    // Unrolled by eight with the group tilings' larger tiles (the limbs'
+    // LDS reads issue under the previous steps' arithmetic; four measured
+    // 2 % slower, sixteen the same).
+    constexpr int RU = TL > TILE ? 8 : 1;
 #pragma unroll RU
     for (uint32_t j = 0; j < len; ++j) {
 #pragma unroll
@@ -1000,10 +1001,10 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         }
         __syncthreads();
         if (ew > 0) {
-          // Unrolled by four in group tilings: the next pivots' LDS reads
+          // Unrolled by eight in group tilings: the next pivots' LDS reads
           // issue under the current ones' multiplications (the one-pair
           // tiling measured slower unrolled).
-          constexpr int RU = B > 1 ? 4 : 1;
+          constexpr int RU = B > 1 ? 8 : 1;
 #pragma unroll RU
           for (uint32_t j = 0; j < len; ++j) {
 #pragma unroll
