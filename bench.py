@@ -73,10 +73,17 @@ def main():
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["headline", "mixed", "skewed", "serve"], default="headline",
+    ap.add_argument("--workload", choices=["headline", "mixed", "skewed", "serve", "single"], default="headline",
                     help="headline: the metric's 100k-bit batch (default); mixed: 4096 pairs of 1k-200k bits, "
                          "autotuned; skewed: 16384 pairs, 95%% 4k-bit and 5%% 500k-bit, 27-bit moduli, replayed "
-                         "hipGraph; serve: Poisson arrivals into the serving loop at five offered rates")
+                         "hipGraph; serve: Poisson arrivals into the serving loop at five offered rates; single: "
+                         "one pair per call beside one-thread mpz_gcd (8k-bit pairs and edge pairs, 1M-bit pairs "
+                         "at L = 32/27/22, 16-352 Kibit sweep)")
+    ap.add_argument("--parts", choices=["all", "small", "large", "sweep"], default="all",
+                    help="single: which part to run")
+    ap.add_argument("--seeds", type=int, default=10, help="single: seeds per configuration")
+    ap.add_argument("--Ls", default="32,27,22", help="single: moduli widths of the 1M-bit part")
+    ap.add_argument("--relay", type=int, default=0, help="single: parts from which the exchange relays (0: default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the modular GCD path is not sharded across GPUs; "
@@ -87,7 +94,8 @@ def main():
     m = ge.package()
     w = ge.workload()
     if args.workload != "headline":
-        {"mixed": run_mixed, "skewed": run_skewed, "serve": run_serve}[args.workload](args, m, w)
+        {"mixed": run_mixed, "skewed": run_skewed, "serve": run_serve,
+         "single": run_single}[args.workload](args, m, w)
         return
     t0 = time.time()
     pairs = w.headline(args.pairs, seed=7, nbits=args.bits, gbits=args.gbits)
@@ -367,6 +375,136 @@ def run_serve(args, m, w):
                                "g = 10% of the length, 32-bit moduli, 64 GB budget, 3 launch slots (one for requests above 16,384 moduli)"},
         "rates": out,
     }), flush=True)
+
+
+# The reference's published one-pair times (ms; ModGCD-OneGPU.pdf p.6, Table 2), random pairs, L = 32.
+PAPER_TABLE2 = {  # Kibit: (GTX 980 Ti, GTX 1080, Tesla P100, Tesla V100, GMP on a Xeon E5-2620)
+    16: (9, 5, 8, 7, 0.6), 32: (16, 10, 15, 13, 1.7), 48: (22, 15, 20, 18, 2.6), 64: (33, 22, 27, 25, 3.8),
+    80: (40, 27, 33, 31, 5.0), 96: (47, 32, 40, 35, 6.5), 112: (53, 38, 47, 40, 7.9), 128: (65, 45, 54, 46, 8.7),
+    144: (78, 50, 61, 52, 10.9), 160: (88, 56, 85, 58, 12.9), 176: (98, 71, 98, 66, 14.9),
+    192: (124, 80, 112, 73, 16.3), 208: (136, 87, 124, 81, 17.8), 224: (152, 103, 138, 94, 17.9),
+    240: (170, 116, 152, 106, 20.1), 256: (199, 125, 164, 118, 22.5), 272: (220, 136, 177, 132, 24.1),
+    288: (238, 158, 189, 143, 25.6), 304: (257, 174, 199, 154, 26.5), 320: (294, 186, 230, 165, 27.3),
+    336: (321, 198, 261, 174, 30.6), 352: (342, 221, 276, 184, 32.9),
+}
+
+
+def _mpz_one_thread(O, m, u, v, reps):
+    """Seconds per mpz_gcd(u, v) on one host thread (mean of reps calls) and the gcd's bit length."""
+    limbs, u_off, nu, v_off, nv = m.pack_pairs([(u, v)] * reps)
+    sec, gbits = O.time_mpz_gcd(limbs, u_off, nu, v_off, nv, 1)
+    return sec / reps, int(gbits[0])
+
+
+def _gpu_one_pair(ctx, u, v, reps):
+    """Median wall seconds of the one-pair call (staging, launch, read-back, conversion), the last
+    call's kernel seconds, its stats and its result."""
+    wall = []
+    g = None
+    for _ in range(reps):
+        t = time.perf_counter()
+        g = ctx.gcd(u, v)
+        wall.append(time.perf_counter() - t)
+    return statistics.median(wall), ctx.last_kernel_seconds(), ctx.last_pair_stats(), g
+
+
+def run_single(args, m, w):
+    """One pair per call (GmpCudaDevice::gcd's path) beside mpz_gcd on one host thread, each pair timed
+    separately, as the reference's harness does (testmodgcd.cpp:261-275):
+      small: 8192-bit pairs (1024-bit odd g), seeds 0..S-1, plus coprime, u = v and v = 0; L = 32;
+      large: ~1M-bit pairs (100k-bit g), seeds 0..S-1, at L = 32, 27, 22;
+      sweep: 16..352 Kibit random pairs, 10 per size, L = 32, beside the reference's published table."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    out = {"metric": "one-pair latency (ms) on one MI355X vs mpz_gcd on one host thread; bit-exact",
+           "unit": "ms", "n_gpus": 1, "higher_is_better": False, "gmp": O.gmp_version(),
+           "cpu_model": _cpu_model(), "timing": "GPU: median wall time of mgcd_gcd_limbs over the repeats "
+           "(limbs staged, launch, digits converted, |G| read back); kernel_ms: the launch's HIP-event time; "
+           "mpz: mean of repeated mpz_gcd calls on one thread, CLOCK_MONOTONIC_RAW"}
+    ctxs = {}
+
+    def ctx_of(L):
+        if L not in ctxs:
+            ctxs[L] = m.Context(L)
+            if args.workgroup and args.slots:
+                ctxs[L].set_option(m.OPT_SINGLE_WORKGROUP, args.workgroup)
+                ctxs[L].set_option(m.OPT_SINGLE_SLOTS, args.slots)
+            if args.relay:
+                ctxs[L].set_option(m.OPT_RELAY_PARTS, args.relay)
+        return ctxs[L]
+
+    def one(L, name, u, v, reps_gpu, reps_cpu):
+        ctx = ctx_of(L)
+        gs, ks, st, g = _gpu_one_pair(ctx, u, v, reps_gpu)
+        cs, gbits = _mpz_one_thread(O, m, u, v, reps_cpu)
+        assert g.bit_length() == gbits and g == O.mpz_gcd(u, v), f"{name}: gcd differs from mpz_gcd"
+        rec = {"pair": name, "bits": max(u.bit_length(), v.bit_length()), "gcd_bits": gbits,
+               "gpu_ms": round(gs * 1e3, 3), "kernel_ms": round(ks * 1e3, 3), "mpz_ms": round(cs * 1e3, 4),
+               "gpu_over_mpz": round(gs / cs, 3) if cs > 0 else None}
+        if v != 0 and u != v:
+            rec.update({"nq": st["nq"], "iters": st["iters"], "digits": st["digits"],
+                        "tiling": list(st["tiling"]), "parts": st["parts"],
+                        "us_per_iter": round(ks * 1e6 / max(1, st["iters"] + st["digits"]), 3)})
+        log(f"L={L} {name}: {rec}")
+        return rec
+
+    def summary(recs):
+        return {"median_gpu_ms": round(statistics.median(r["gpu_ms"] for r in recs), 3),
+                "median_mpz_ms": round(statistics.median(r["mpz_ms"] for r in recs), 4),
+                "gpu_faster_than_mpz": sum(r["gpu_ms"] < r["mpz_ms"] for r in recs), "pairs": recs}
+
+    if args.parts in ("all", "small"):
+        u, v = w.single_small(0)
+        ctx_of(32).gcd(u, v)  # first call: buffers and code objects
+        recs = [one(32, f"seed {s}", *w.single_small(s), 5, 200) for s in range(args.seeds)]
+        edges = w.single_small_edges(0)
+        recs += [one(32, name, u, v, 5, 200) for name, (u, v) in zip(("coprime", "u == v", "v == 0"), edges)]
+        out["small_8192"] = {"L": 32, "config": "g 1024-bit odd, a, b 7168-bit, u = g a, v = g b; edge pairs",
+                             **summary(recs)}
+    if args.parts in ("all", "large"):
+        res = {}
+        for L in [int(x) for x in args.Ls.split(",")]:
+            u, v = w.single_large(0)
+            recs = [one(L, f"seed {s}", *w.single_large(s), 1, 1) for s in range(args.seeds)]
+            res[str(L)] = summary(recs)
+        out["large_1M"] = {"config": "g 100k-bit, a, b 900k-bit, u = g a, v = g b (~1M bits)", "by_L": res}
+    if args.parts in ("all", "sweep"):
+        rows = []
+        ctx = ctx_of(32)
+        for kib in sorted(PAPER_TABLE2):
+            nbits = kib * 1024
+            ps = [w.pair(w.Stream(8, kib * 100 + i), nbits, 0, "coprime") for i in range(10)]
+            gms, kms, cms = [], [], []
+            for u, v in ps:
+                gs, ks, st, g = _gpu_one_pair(ctx, u, v, 1)
+                cs, gbits = _mpz_one_thread(O, m, u, v, 3)
+                assert g.bit_length() == gbits and g == math.gcd(u, v), f"{kib} Kibit: gcd differs"
+                gms.append(gs)
+                kms.append(ks)
+                cms.append(cs)
+            paper = PAPER_TABLE2[kib]
+            rows.append({"kibit": kib, "gpu_ms": round(statistics.mean(gms) * 1e3, 3),
+                         "kernel_ms": round(statistics.mean(kms) * 1e3, 3),
+                         "mpz_ms": round(statistics.mean(cms) * 1e3, 3),
+                         "paper_v100_ms": paper[3], "paper_gtx1080_ms": paper[1], "paper_gmp_ms": paper[4],
+                         "nq": st["nq"], "parts": st["parts"]})
+            log(f"sweep {kib} Kibit: {rows[-1]}")
+        out["sweep_kibit"] = {"L": 32, "config": "random pairs (gcd ~ 1), 10 per size, mean; paper: PDF p.6 "
+                                                 "Table 2 (mean of 10 pairs per size)", "rows": rows}
+    for c in ctxs.values():
+        c.close()
+    print(json.dumps(out), flush=True)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 if __name__ == "__main__":

@@ -131,6 +131,13 @@ int mgcd_mixed_radix_to_limbs(mgcd_ctx* ctx, const mgcd_pair_t* digits, size_t k
 int mgcd_ctx_synchronize(mgcd_ctx* ctx);
 /* Seconds the last gcd kernel launch of this context took (HIP events). */
 double mgcd_ctx_last_kernel_seconds(const mgcd_ctx* ctx);
+/* Work of the last one-pair launch of this context (mgcd_gcd_limbs,
+ * mgcd_gcd_limbs_trace): N_Q, elimination iterations R, recovery digits k,
+ * and its tiling (threads per workgroup, moduli slots per thread,
+ * workgroups).  Any pointer may be NULL.  No counterpart in the reference,
+ * whose harness times the call only (testmodgcd.cpp:265-275). */
+int mgcd_ctx_last_pair_stats(const mgcd_ctx* ctx, uint32_t* nq, uint32_t* iters, uint32_t* digits, int* workgroup,
+                             int* slots, int* parts);
 
 /* One pair, one launch sequence: the path of GmpCudaDevice::gcd
  * (GmpCudaDevice-gcd.cu:57-132) without GMP.  u, v may come in either order

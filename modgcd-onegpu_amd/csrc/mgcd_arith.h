@@ -37,9 +37,11 @@
 // less than 1/4 (exact when y is a power of two), so |r| < 3|y|/4; with
 // |r_i s_(i+1) - r_(i+1) s_i| = m the cofactors then stay below 1.6 m, and
 // every FP32 FMA of the loop is exact.
+//
+// The header needs no HIP include: hipcc provides the device qualifiers and
+// builtins, and the CPU tests build it with g++ and a model of v_rcp_f32.
 #pragma once
 
-#include <hip/hip_runtime.h>
 #include <cstdint>
 
 namespace mgcd {

@@ -43,19 +43,16 @@
 
 #include "mgcd_arith.h"
 #include "mgcd_kernels.h"
+#include "mgcd_keys.h"
 
 namespace mgcd {
 using namespace dev;
+using namespace keys;
 
 namespace {
 
-constexpr int TILE = 1024;                      // doubles of LDS staging (limbs, or 512 pivots) of
-                                                // the one-pair tilings; batch tilings stage more
-constexpr uint32_t IDX_MAX = (1u << 19) - 1;    // moduli indices < 2^19
-constexpr uint64_t PAYLOAD = (1ull << 52) - 1;  // slot bits below the tag
-constexpr uint32_t NONE_IDX = 0xFFFFFu;
-constexpr uint64_t EXP52 = 0x4330000000000000ull;  // bit pattern of 2^52
-
+constexpr int TILE = 1024;  // doubles of LDS staging (limbs, or 512 pivots) of the one-pair
+                            // tilings; batch tilings stage more
 // t sentinels.  Finite t are the reference's toSigned values, |t| < 2^31.
 constexpr double TINF = 2147483648.0;    // vq == 0: the reference's MOD_INFINITY
 constexpr double TZERO = -2147483648.0;  // uq == vq == 0
@@ -64,34 +61,6 @@ constexpr double TFIN = 2147483648.0;    // |t| < TFIN: finite
 // Within one round only: a finite t equal to its pair's pivot (t - b == 0
 // mod m), whose next state is TINF.  Same high word as TINF, low word 1.
 constexpr double TPOLE = 2147483648.0 + 0x1p-21;
-
-// Keys are doubles holding exact integers, compared with v_min_f64:
-//   elimination  |t| * 2^20 + 2 (IDX_MAX - idx) + [t < 0]
-//   recovery     idx * 2^32 + (value + 2^31)
-// For moduli in descending index order the elimination key orders exactly
-// like the reference's (|t| << 32) | (m - [t >= 0]) (getGcdKernel.cu:154);
-// |t| = 2^31 (TINF, TZERO) is its infinity.  KINF caps the keys of
-// retired lanes so that every key fits the 52-bit slot payload.
-constexpr double KINF = 2251799814733823.0;  // 2^51 + 2^20 - 1
-constexpr double KNONE = __builtin_huge_val();  // no result (a wait timed out)
-
-// |t - 2^-21| * 2^20 is |t| * 2^20 - 1/2 for t > 0 and |t| * 2^20 + 1/2
-// for t <= 0 (exact: t is an integer below 2^32); ck = 1/2 + 2 (IDX_MAX -
-// idx) completes the key.  t = 0 gets sign bit 1, which decodes to 0 too.
-__device__ __forceinline__ double elim_key(double t, double ck) {
-  return __builtin_fma(__builtin_fabs(t - 0x1p-21), 0x1p20, ck);
-}
-__device__ __forceinline__ uint64_t key_bits(double k) {  // exact integer of a key < 2^52
-  return __builtin_bit_cast(uint64_t, k + 0x1p52) & PAYLOAD;
-}
-__device__ __forceinline__ double key_from_bits(uint64_t x) {
-  return __builtin_bit_cast(double, x | EXP52) - 0x1p52;
-}
-__device__ __forceinline__ void elim_decode(uint64_t key, uint32_t& idx, int32_t& val) {
-  const uint32_t a = (uint32_t)(key >> 20);
-  idx = IDX_MAX - (uint32_t)((key >> 1) & IDX_MAX);
-  val = (key & 1) ? (int32_t)(0u - a) : (int32_t)a;  // |t| = 2^31 decodes to INT32_MIN
-}
 
 // The reference's toSigned (getGcdKernel.cu:278-281) of an exact integer
 // r, |r| < 2^40: the representative in [-(m+1)/2, (m-3)/2].  rint((r+1)/m)
@@ -276,33 +245,46 @@ struct Shared {
 struct Ctx {
   uint32_t G, g;             // parts of the group, this part
   uint32_t rnd;              // cross-workgroup round counter
+  uint32_t phase;            // 0: elimination rounds; 1: the rounds after them
   unsigned long long* bar;   // this group's 2*(G+1)*B exchange slots, then G residue-job words
   uint32_t spin_limit;
   bool relay;                // exchange through part 0 (large G)
+  bool abort;                // a wait of this part timed out, or a peer's did: post ABORT from now on
 };
 
 // Cross-workgroup minima, called by wave 0 (all 64 lanes): lane b < B
 // brings this workgroup's value for pair b; lane l gets the group's minimum
-// for pair l % B, KNONE for every pair when a wait timed out.  Slot layout
-// of a row: [G][B], so lane l polls only slots of pair l % B.  Every part
-// polls the whole row (one hop), or, for groups of many parts, part 0 polls
-// it and posts the minima into a relay row of B slots that the others poll
-// (two hops, but G*B + (G-1)*B polled slots per round instead of G*G*B).
+// for pair l % B, KNONE for every pair when a wait timed out or a peer
+// posted ABORT.  Slot layout of a row: [G][B], so lane l polls only slots of
+// pair l % B.  Every part polls the whole row (one hop), or, for groups of
+// many parts, part 0 polls it and posts the minima into a relay row of B
+// slots that the others poll (two hops, but G*B + (G-1)*B polled slots per
+// round instead of G*G*B).
+//
+// Tag: an 11-bit round number (never 0, so cleared slots never match) and
+// the phase bit.  Once a part has given up it posts ABORT in every later
+// round, whatever it would have posted, so a peer still waiting for it
+// gives up too instead of reading a value of another kind.
 template <int B>
 __device__ double grid_min(double x, Ctx& C) {
   const uint32_t lane = __lane_id();
   C.rnd += 1;
-  const uint64_t tag = (uint64_t)((C.rnd - 1) % 4095u + 1u);
+  const uint64_t tag = (uint64_t)(((C.rnd - 1) % 2047u + 1u) | (C.phase << 11));
+  // A slot is ready when its round matches; a matching round of the other
+  // phase means a peer has left the sequence, and counts as ABORT.
+  auto ready = [&](uint64_t v) { return ((v >> 52) & 0x7FFu) == (tag & 0x7FFu); };
+  auto aborted = [&](uint64_t v) { return (v >> 52) != tag || (v & PAYLOAD) == ABORT; };
   unsigned long long* row = C.bar + (size_t)(C.rnd & 1) * C.G * B;
   unsigned long long* relay = C.bar + (size_t)2 * C.G * B + (size_t)(C.rnd & 1) * B;
   if (lane < (uint32_t)B)
-    __hip_atomic_store(row + (size_t)C.g * B + lane, (unsigned long long)((tag << 52) | key_bits(x)),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(row + (size_t)C.g * B + lane,
+                       (unsigned long long)((tag << 52) | (C.abort ? ABORT : key_bits(x))), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   if (C.relay && C.g != 0) {
     uint64_t v = __hip_atomic_load(relay + lane % B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t spins = 0;
     bool late = false;
-    while ((v >> 52) != tag) {
+    while (!ready(v)) {
       if (++spins > C.spin_limit) {
         late = true;
         break;
@@ -310,12 +292,15 @@ __device__ double grid_min(double x, Ctx& C) {
       __builtin_amdgcn_s_sleep(1);
       v = __hip_atomic_load(relay + lane % B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // A relayed KNONE (part 0 timed out) arrives as the all-ones payload.
-    const bool none = (v & PAYLOAD) == PAYLOAD;
-    return (__ballot(late) || __ballot(none)) ? KNONE : key_from_bits(v & PAYLOAD);
+    const bool none = !late && aborted(v);
+    if (__ballot(late) || __ballot(none)) {
+      C.abort = true;
+      return KNONE;
+    }
+    return key_from_bits(v & PAYLOAD);
   }
   double m = KNONE;
-  bool late = false;
+  bool late = false, none = false;
   const uint32_t nslots = C.G * B;
   // Up to PL slots per lane per pass: their loads are in flight together.
   // Several passes (groups of many parts): every slot not yet posted is
@@ -327,21 +312,21 @@ __device__ double grid_min(double x, Ctx& C) {
   // slower: 10 % on 45-part groups.)
   constexpr int PL = 4;
   if (nslots > PL * 64) {
-    for (uint32_t j0 = lane; j0 < nslots; j0 += PL * 64) {
+    for (uint32_t j0 = lane; j0 < nslots && !late; j0 += PL * 64) {
       uint64_t v[PL];
       bool pend[PL];
 #pragma unroll
       for (int q = 0; q < PL; ++q) {
         const uint32_t j = j0 + 64 * q;
         pend[q] = j < nslots;
-        v[q] = pend[q] ? __hip_atomic_load(row + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag << 52) | PAYLOAD;
+        v[q] = pend[q] ? __hip_atomic_load(row + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag << 52) | NEUTRAL;
       }
       uint32_t spins = 0;
       for (;;) {
         bool any = false;
 #pragma unroll
         for (int q = 0; q < PL; ++q) {
-          pend[q] = pend[q] && (v[q] >> 52) != tag;
+          pend[q] = pend[q] && !ready(v[q]);
           any |= pend[q];
         }
         if (!any) break;
@@ -355,7 +340,10 @@ __device__ double grid_min(double x, Ctx& C) {
           if (pend[q]) v[q] = __hip_atomic_load(row + j0 + 64 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
 #pragma unroll
-      for (int q = 0; q < PL; ++q) m = __builtin_fmin(m, key_from_bits(v[q] & PAYLOAD));
+      for (int q = 0; q < PL; ++q) {
+        none |= aborted(v[q]);
+        m = __builtin_fmin(m, key_from_bits(v[q] & PAYLOAD));
+      }
     }
   } else {
     const uint32_t j0 = lane;
@@ -363,12 +351,12 @@ __device__ double grid_min(double x, Ctx& C) {
 #pragma unroll
     for (int q = 0; q < PL; ++q) {
       const uint32_t j = j0 + 64 * q;
-      v[q] = j < nslots ? __hip_atomic_load(row + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag << 52) | PAYLOAD;
+      v[q] = j < nslots ? __hip_atomic_load(row + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag << 52) | NEUTRAL;
     }
 #pragma unroll
     for (int q = 0; q < PL; ++q) {
       uint32_t spins = 0;
-      while ((v[q] >> 52) != tag) {
+      while (!ready(v[q])) {
         if (++spins > C.spin_limit) {
           late = true;
           break;
@@ -376,15 +364,20 @@ __device__ double grid_min(double x, Ctx& C) {
         __builtin_amdgcn_s_sleep(1);
         v[q] = __hip_atomic_load(row + j0 + 64 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+      none |= aborted(v[q]);
       m = __builtin_fmin(m, key_from_bits(v[q] & PAYLOAD));
     }
   }
   m = class_min<B>(m);
-  const bool any_late = __ballot(late) != 0;
+  const bool give_up = (__ballot(late) | __ballot(none)) != 0;
   if (C.relay && lane < (uint32_t)B)
-    __hip_atomic_store(relay + lane, (unsigned long long)((tag << 52) | (any_late ? PAYLOAD : key_bits(m))),
+    __hip_atomic_store(relay + lane, (unsigned long long)((tag << 52) | (give_up ? ABORT : key_bits(m))),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return any_late ? KNONE : m;
+  if (give_up) {
+    C.abort = true;
+    return KNONE;
+  }
+  return m;
 }
 
 // (a) Residues of NOP operands at once (x[o], n[o] limbs) for every slot
@@ -714,13 +707,14 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
     const double kr = reduce_keys(wkey);
     if (wave == 0) {
       if (lane < npairs) {
-        if (kr == KNONE) {
+        uint32_t idx;
+        int32_t val;
+        // An infinite first pivot is recorded too; a malformed key (only
+        // ever after a peer gave up) ends the pair like a timeout.
+        if (kr == KNONE || decode_pivot(kr, nq, idx, val) == KEY_BAD) {
           ps.status = KST_TIMEOUT;
           ps.live = 0;
         } else {
-          uint32_t idx;
-          int32_t val;
-          elim_decode(key_bits(kr), idx, val);
           publish_pivot(args, lead, lds_mods, lane, ps, idx, val, S);
           mark_retire(lane, idx);
         }
@@ -878,14 +872,14 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         if (ps.remaining * (L - 2) <= ps.ubits) {                   // :393-399
           ps.status = KST_REDUX;
           ps.live = 0;
-        } else if (kr == KNONE) {
-          ps.status = KST_TIMEOUT;
-          ps.live = 0;
         } else {
           uint32_t idx;
           int32_t val;
-          elim_decode(key_bits(kr), idx, val);
-          if (val == INT32_MIN) {
+          const int kind = kr == KNONE ? KEY_BAD : decode_pivot(kr, nq, idx, val);
+          if (kind == KEY_BAD) {
+            ps.status = KST_TIMEOUT;
+            ps.live = 0;
+          } else if (kind == KEY_INF) {
             ps.live = 0;  // infinity: done (:402)
           } else {
             publish_pivot(args, lead, lds_mods, lane, ps, idx, val, S);
@@ -921,6 +915,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
     S.ntrace[lane] = ps.niter;
   }
   // The pivots recorded by the leader workgroup become visible to all parts.
+  C.phase = 1;
   if (C.G > 1 && wave == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     (void)grid_min<B>(0.0, C);
@@ -1090,7 +1085,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
           if (((am[j] >> k) & 1) && (double)(base + (uint32_t)k * T + tid) == x) own[k] |= 1u << j;
         // No nonzero residue in this part: (NONE_IDX, 0), which the reference
         // resolves to its thread 0 pair with value 0 (:57-64).
-        if ((uint32_t)j == tid && x == (double)NONE_IDX) S.sel[j] = (double)NONE_IDX * 0x1p32 + 0x1p31;
+        if ((uint32_t)j == tid && x == (double)NONE_IDX) S.sel[j] = KNODIGIT;
       }
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -1130,15 +1125,6 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       S.pidx[lane - c0] = idx;
       ps.pval = val;
     };
-    auto decode_sel = [&](double x, uint32_t& idx, int32_t& val) {
-      const uint64_t kb = key_bits(x);
-      idx = (uint32_t)(kb >> 32);
-      val = (int32_t)((uint32_t)kb - 0x80000000u);
-      if (idx == NONE_IDX) {
-        idx = 0;
-        val = 0;
-      }
-    };
     // Pairs not (or no longer) in recovery publish an update that matches
     // no modulus.
     auto finish_round = [&]() {
@@ -1161,13 +1147,12 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
     {
       const double x = select();
       if (mine && lane < npairs && ps.live) {
-        if (x == KNONE) {
+        uint32_t idx;
+        int32_t val;
+        if (x == KNONE || decode_digit(x, nq, idx, val) == KEY_BAD) {
           ps.status = KST_TIMEOUT;
           ps.live = 0;
         } else {
-          uint32_t idx;
-          int32_t val;
-          decode_sel(x, idx, val);
           set_digit(idx, val);
         }
       }
@@ -1204,15 +1189,17 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         if (ps.remaining <= 0) {                                 // :429-430, :440
           ps.status = ps.pval != 0 ? KST_RECOVERY : KST_OK;
           ps.live = 0;
-        } else if (x == KNONE) {
-          ps.status = KST_TIMEOUT;
-          ps.live = 0;
         } else {
           uint32_t idx;
           int32_t val;
-          decode_sel(x, idx, val);
-          if (val == 0) ps.live = 0;  // :433
-          else set_digit(idx, val);
+          if (x == KNONE || decode_digit(x, nq, idx, val) == KEY_BAD) {
+            ps.status = KST_TIMEOUT;
+            ps.live = 0;
+          } else if (val == 0) {
+            ps.live = 0;  // :433
+          } else {
+            set_digit(idx, val);
+          }
         }
       }
       finish_round();
@@ -1262,6 +1249,8 @@ __global__ __launch_bounds__(T) void gcd_kernel(KernelArgs args) {
     C.G = Gd.G;
     C.g = W.part;
     C.rnd = 0;
+    C.phase = 0;
+    C.abort = false;
     C.relay = Gd.G >= args.relay_min_g;
     C.bar = args.bar + Gd.bar_off;
     process_group<T, K, B, A>(args, Gd, S, C);

@@ -297,6 +297,9 @@ struct mgcd_ctx {
   // pooled buffers of the single-pair path
   DevBuf limbs, pairs, groups, items, queue, bar, digits, trace, res, out;
   double last_kernel_s = 0.0;
+  // the last one-pair launch: N_Q, iterations, digits, tiling (mgcd_ctx_last_pair_stats)
+  uint32_t last_nq = 0, last_iters = 0, last_digits = 0;
+  int last_T = 0, last_K = 0, last_G = 0;
   uint32_t force_nq = 0;  // MGCD_OPT_FORCE_MODULI
   bool retry = true;      // MGCD_OPT_RETRY
   int single_T = 0;       // MGCD_OPT_SINGLE_WORKGROUP (0: automatic)
@@ -732,6 +735,12 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
   HIP_TRY(hipStreamSynchronize(s));
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_kernel_s = ms * 1e-3;
+  ctx->last_nq = nq;
+  ctx->last_iters = po.ntrace;
+  ctx->last_digits = po.ndig;
+  ctx->last_T = kernel_variant(vi).T;
+  ctx->last_K = kernel_variant(vi).K;
+  ctx->last_G = (int)groups[0].G;
   if (nt) *nt = po.ntrace;
   if (nd) *nd = po.ndig;
   if (trace_cap && po.ntrace)
@@ -821,6 +830,18 @@ int mgcd_gcd_limbs(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* 
 }
 
 double mgcd_ctx_last_kernel_seconds(const mgcd_ctx* ctx) { return ctx ? ctx->last_kernel_s : 0.0; }
+
+int mgcd_ctx_last_pair_stats(const mgcd_ctx* ctx, uint32_t* nq, uint32_t* iters, uint32_t* digits, int* workgroup,
+                             int* slots, int* parts) {
+  if (!ctx) return MGCD_ERR_ARG;
+  if (nq) *nq = ctx->last_nq;
+  if (iters) *iters = ctx->last_iters;
+  if (digits) *digits = ctx->last_digits;
+  if (workgroup) *workgroup = ctx->last_T;
+  if (slots) *slots = ctx->last_K;
+  if (parts) *parts = ctx->last_G;
+  return MGCD_OK;
+}
 
 int mgcd_ctx_synchronize(mgcd_ctx* ctx) {
   if (!ctx) return MGCD_ERR_ARG;

@@ -38,7 +38,7 @@ EXPORTS = [
     "mgcd_certify_quasi_quotient", "mgcd_last_error_detail", "mgcd_ctx_last_kernel_seconds",
     "mgcd_ctx_synchronize", "mgcd_batch_pair_stats", "mgcd_ctx_set_option", "mgcd_batch_launch_count",
     "mgcd_batch_launch_info", "mgcd_autotune", "mgcd_server_create", "mgcd_server_submit", "mgcd_server_wait",
-    "mgcd_server_stats", "mgcd_server_destroy", "mgcd_mixed_radix_to_limbs",
+    "mgcd_server_stats", "mgcd_server_destroy", "mgcd_mixed_radix_to_limbs", "mgcd_ctx_last_pair_stats",
 ]
 
 OPT_FORCE_MODULI = 1
@@ -107,6 +107,8 @@ def lib():
         L.mgcd_ctx_last_kernel_seconds.argtypes = [ctypes.c_void_p]
         L.mgcd_ctx_last_kernel_seconds.restype = ctypes.c_double
         L.mgcd_ctx_synchronize.argtypes = [ctypes.c_void_p]
+        L.mgcd_ctx_last_pair_stats.argtypes = [ctypes.c_void_p, P32, P32, P32, ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.mgcd_ctx_set_option.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]
         L.mgcd_batch_pair_stats.argtypes = [ctypes.c_void_p, ctypes.c_size_t, P32, P32, P32]
         L.mgcd_batch_launch_count.argtypes = [ctypes.c_void_p, PSZ]
@@ -252,6 +254,14 @@ class Context:
 
     def last_kernel_seconds(self):
         return lib().mgcd_ctx_last_kernel_seconds(self.h)
+
+    def last_pair_stats(self):
+        """dict(nq, iters, digits, tiling=(T, K), parts) of the last one-pair launch."""
+        q, r, k = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        t, s, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().mgcd_ctx_last_pair_stats(self.h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(k),
+                                             ctypes.byref(t), ctypes.byref(s), ctypes.byref(g)))
+        return {"nq": q.value, "iters": r.value, "digits": k.value, "tiling": (t.value, s.value), "parts": g.value}
 
     def certify(self, mode, L=0):
         fail, checked = ctypes.c_uint64(), ctypes.c_uint64()
