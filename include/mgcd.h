@@ -36,7 +36,8 @@ typedef enum {
   MGCD_ERR_CAPACITY = 10,          /* caller's output buffer too small */
   MGCD_ERR_NOMEM = 11,             /* host or device allocation failed */
   MGCD_ERR_TIMEOUT = 12,           /* a bounded in-kernel wait expired */
-  MGCD_ERR_BUSY = 13               /* server: admission budget exhausted */
+  MGCD_ERR_BUSY = 13,              /* server: admission budget exhausted */
+  MGCD_ERR_SCRATCH = 14            /* input beyond the limit of mgcd_gcd_limbs_noalloc */
 } mgcd_status;
 
 /* One signed mixed-radix digit: the pair_t the reference kernel hands back
@@ -145,6 +146,20 @@ int mgcd_ctx_last_pair_stats(const mgcd_ctx* ctx, uint32_t* nq, uint32_t* iters,
  * *ng (0 for a zero gcd).  Blocking. */
 int mgcd_gcd_limbs(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv,
                    uint32_t* g, size_t cap, size_t* ng);
+
+/* As mgcd_gcd_limbs, without allocating: no hipMalloc, hipFree or other
+ * allocating call.  No counterpart in the reference; it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid it).  Every device buffer of the call lives
+ * in a fixed 2 MiB array of the library that each device holds until the
+ * process ends; its address is taken when a context is first created on the
+ * device.  Limits: N_Q <= 16384 moduli (operands up to about 176 kbit at
+ * L = 32, 153 kbit at L = 22; retries stay within it) and nu + nv <= 12288
+ * limbs; larger inputs return MGCD_ERR_SCRATCH.  Calls on one device share
+ * that array: calls that overlap on one device (from several threads or
+ * contexts) race, so they must be serialised by the caller. */
+int mgcd_gcd_limbs_noalloc(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv,
+                           uint32_t* g, size_t cap, size_t* ng);
 
 /* As mgcd_gcd_limbs, and also returns the kernel's signed mixed-radix
  * digits (least significant first) and the elimination pivots (p, b) in

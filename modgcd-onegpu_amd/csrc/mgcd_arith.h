@@ -38,8 +38,9 @@
 // |r_i s_(i+1) - r_(i+1) s_i| = m the cofactors then stay below 1.6 m, and
 // every FP32 FMA of the loop is exact.
 //
-// The header needs no HIP include: hipcc provides the device qualifiers and
-// builtins, and the CPU tests build it with g++ and a model of v_rcp_f32.
+// The header includes no HIP header: its includer provides the device
+// qualifiers (hip_runtime.h), and the CPU tests build it with g++ and a
+// model of v_rcp_f32.
 #pragma once
 
 #include <cstdint>

@@ -41,8 +41,8 @@
 // hanging.
 #include <type_traits>
 
+#include "mgcd_kernels.h"  // first: the HIP runtime header, which the two below rely on
 #include "mgcd_arith.h"
-#include "mgcd_kernels.h"
 #include "mgcd_keys.h"
 
 namespace mgcd {

@@ -49,6 +49,7 @@ const char* kMessages[] = {
     "Out of memory.",
     "A device-side wait timed out.",
     "Admission budget exhausted.",
+    "Input exceeds the no-allocation scratch limit.",
 };
 
 thread_local std::string g_last_error;
@@ -280,6 +281,78 @@ int kst_to_status(int32_t k) {
     case KST_TIMEOUT: return MGCD_ERR_TIMEOUT;
     default: return MGCD_ERR_KERNEL;
   }
+}
+
+// ---------------------------------------------------------------------------
+// The one-pair path without allocation (mgcd_gcd_limbs_noalloc).  Every
+// device buffer of such a call lives in `scratch`, a file-scope device array
+// that each device the library runs on holds until the process ends.  Its
+// name is a host-side stand-in, not a device address: host code uses only
+// the address hipGetSymbolAddress returns for the current device, taken once
+// per device when a context is created there (scratch_init).  Calls share
+// the array of their device and keep nothing in it between calls; calls that
+// overlap on one device race.
+
+constexpr uint32_t kScratchMaxModuli = 16384;  // N_Q (~176 kbit operands at L = 32, ~153 kbit at L = 22)
+constexpr size_t kScratchMaxLimbs = 12288;     // nu + nv
+constexpr uint32_t kScratchMaxParts = 64;      // workgroups of one pair (N_Q / 256)
+constexpr uint32_t kScratchMaxLevels = 32;     // product-tree levels of the conversion
+constexpr size_t kScratchBytes = (size_t)2 << 20;
+
+struct ScratchLayout {
+  size_t limbs, pair, group, items, queue, bar, digits, trace, res, out, mrpair, mrcum, buf0, buf1, mres, mlen, end;
+};
+constexpr size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr ScratchLayout scratch_layout() {
+  ScratchLayout l{};
+  size_t o = 0;
+  l.limbs = o;  o = al256(o + kScratchMaxLimbs * 4);
+  l.pair = o;   o = al256(o + sizeof(PairDesc));
+  l.group = o;  o = al256(o + sizeof(GroupDesc));
+  l.items = o;  o = al256(o + kScratchMaxParts * sizeof(WorkItem));
+  l.queue = o;  o = al256(o + (size_t)kMaxQueues * kQueueStride * 4);
+  l.bar = o;    o = al256(o + ((size_t)2 * (kScratchMaxParts + 1) + kScratchMaxParts) * 8);  // one pair per group
+  l.digits = o; o = al256(o + (size_t)kScratchMaxModuli * 8);
+  l.trace = o;  o = al256(o + ((size_t)kScratchMaxModuli + 32) * 8);
+  l.res = o;    o = al256(o + ((size_t)2 * kScratchMaxModuli + 64) * 4);
+  l.out = o;    o = al256(o + sizeof(PairOut));
+  l.mrpair = o; o = al256(o + sizeof(MrPair));
+  l.mrcum = o;  o = al256(o + (size_t)kScratchMaxLevels * 2 * 4);
+  l.buf0 = o;   o = al256(o + (8 * (size_t)kScratchMaxModuli + 8) * 4);
+  l.buf1 = o;   o = al256(o + (8 * (size_t)kScratchMaxModuli + 8) * 4);
+  l.mres = o;   o = al256(o + ((size_t)kScratchMaxModuli + 1) * 4);
+  l.mlen = o;   o = al256(o + 4);
+  l.end = o;
+  return l;
+}
+constexpr ScratchLayout kScratch = scratch_layout();
+static_assert(kScratch.end <= kScratchBytes, "the scratch layout must fit the array");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kScratchDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current.
+int scratch_init(int device) {
+  if (device < 0 || device >= kScratchDevices) return MGCD_ERR_INVALID_DEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return MGCD_OK;
+  void* p = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) {
+    g_last_error = "hipGetSymbolAddress gave no address for the scratch array";
+    return MGCD_ERR_HIP;
+  }
+  g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  return MGCD_OK;
+}
+
+unsigned char* scratch_of(int device) {
+  if (device < 0 || device >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
 }
 
 }  // namespace
@@ -552,6 +625,13 @@ int mgcd_ctx_create(mgcd_ctx** out, int device, int L, const char* table_path) {
     return MGCD_ERR_HIP;
   }
   ctx->num_cus = props.multiProcessorCount;
+  {
+    const int src = scratch_init(device);
+    if (src != MGCD_OK) {
+      delete ctx;
+      return src;
+    }
+  }
   const size_t nm = ctx->moduli.size();
   std::vector<double> md(nm), mi(nm);
   for (size_t i = 0; i < nm; ++i) {
@@ -768,6 +848,125 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
   return MGCD_OK;
 }
 
+// gcd_single with every device buffer in the scratch array and no host
+// container: one pair, its G parts, one queue; digits converted on the
+// device by the product tree, |G| copied straight into g.
+int gcd_single_noalloc(mgcd_ctx* ctx, unsigned char* base, const uint32_t* u, size_t nu, const uint32_t* v,
+                       size_t nv, uint32_t nq, uint32_t* g, size_t cap, size_t* ng) {
+  *ng = 0;
+  if (nq == 0 || nq > ctx->moduli.size()) return MGCD_ERR_NOT_ENOUGH_MODULI;
+  if (nq > kScratchMaxModuli || nu + nv > kScratchMaxLimbs) return MGCD_ERR_SCRATCH;
+  const int vi = choose_single_variant(ctx, nq);
+  if (vi < 0) return MGCD_ERR_TOO_MANY_THREADS;
+  const KernelVariant& kv = kernel_variant(vi);
+  const uint32_t G = (nq + (uint32_t)(kv.T * kv.K) - 1) / (uint32_t)(kv.T * kv.K);
+  if (kv.B != 1 || G > kScratchMaxParts) return MGCD_ERR_SCRATCH;
+  if (G > (uint32_t)ctx->max_resident[vi]) return MGCD_ERR_TOO_MANY_THREADS;
+
+  PairDesc pd;
+  std::memset(&pd, 0, sizeof pd);
+  pd.u_off = 0;
+  pd.v_off = nu;
+  pd.nu = (uint32_t)nu;
+  pd.nv = (uint32_t)nv;
+  pd.nq = nq;
+  pd.trace_cap = nq + 1;
+  GroupDesc gd;
+  std::memset(&gd, 0, sizeof gd);
+  gd.count = 1;
+  gd.nq = nq;
+  gd.G = G;
+  WorkItem items[kScratchMaxParts];
+  for (uint32_t i = 0; i < G; ++i) items[i] = WorkItem{0, i};
+  LaunchSet ls;
+  ls.variant = vi;
+  ls.nitems = G;
+  ls.nqueues = 1;
+  ls.qoff[0] = 0;
+  ls.qoff[1] = G;
+  ls.grid = G;
+  ls.nbar = G > 1 ? 2 * ((size_t)G + 1) + G : 0;
+
+  hipStream_t s = ctx->stream;
+  uint32_t* d_limbs = reinterpret_cast<uint32_t*>(base + kScratch.limbs);
+  if (nu) HIP_TRY(hipMemcpyAsync(d_limbs, u, nu * 4, hipMemcpyHostToDevice, s));
+  if (nv) HIP_TRY(hipMemcpyAsync(d_limbs + nu, v, nv * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(base + kScratch.pair, &pd, sizeof pd, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(base + kScratch.group, &gd, sizeof gd, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(base + kScratch.items, items, G * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+
+  KernelArgs ka;
+  std::memset(&ka, 0, sizeof ka);
+  ka.limbs = d_limbs;
+  ka.pairs = reinterpret_cast<const PairDesc*>(base + kScratch.pair);
+  ka.groups = reinterpret_cast<const GroupDesc*>(base + kScratch.group);
+  ka.items = reinterpret_cast<const WorkItem*>(base + kScratch.items);
+  ka.L = ctx->L;
+  ka.queue = reinterpret_cast<uint32_t*>(base + kScratch.queue);
+  ka.mod_u32 = ctx->d_mod_u32.as<uint32_t>();
+  ka.mod_d = ctx->d_mod_d.as<double>();
+  ka.mod_inv = ctx->d_mod_inv.as<double>();
+  ka.bar = reinterpret_cast<unsigned long long*>(base + kScratch.bar);
+  ka.digits = reinterpret_cast<mgcd_pair_t*>(base + kScratch.digits);
+  ka.trace = reinterpret_cast<mgcd_pair_t*>(base + kScratch.trace);
+  ka.res = reinterpret_cast<uint32_t*>(base + kScratch.res);
+  ka.out = reinterpret_cast<PairOut*>(base + kScratch.out);
+  ka.spin_limit = ctx->spin_limit;
+  ka.relay_min_g = ctx->relay_min_g;
+  int rc = launch(ctx, ls, ka);  // clears the queue counters and exchange slots with fill kernels
+  if (rc != MGCD_OK) return rc;
+  PairOut po;
+  HIP_TRY(hipMemcpyAsync(&po, ka.out, sizeof po, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_kernel_s = ms * 1e-3;
+  if (po.status != KST_OK) return kst_to_status(po.status);
+  if (po.ndig > nq) return MGCD_ERR_KERNEL;
+  const uint32_t k = po.ndig;
+  if (k == 0) return MGCD_OK;
+
+  // Digits -> |G| (the product tree of mr_convert, for one pair).
+  MrPair mp;
+  std::memset(&mp, 0, sizeof mp);
+  mp.k = k;
+  mp.root_level = ceil_log2(k);
+  const uint32_t levels = mp.root_level;
+  if (levels > kScratchMaxLevels) return MGCD_ERR_SCRATCH;
+  uint32_t cum[kScratchMaxLevels * 2];
+  for (uint32_t j = 0; j < levels; ++j) {
+    cum[2 * j] = 0;
+    cum[2 * j + 1] = (uint64_t)k > (1ull << j) ? (uint32_t)((k + (2ull << j) - 1) >> (j + 1)) : 0u;
+  }
+  HIP_TRY(hipMemcpyAsync(base + kScratch.mrpair, &mp, sizeof mp, hipMemcpyHostToDevice, s));
+  if (levels) HIP_TRY(hipMemcpyAsync(base + kScratch.mrcum, cum, levels * 2 * 4, hipMemcpyHostToDevice, s));
+  uint32_t* lv[2] = {reinterpret_cast<uint32_t*>(base + kScratch.buf0), reinterpret_cast<uint32_t*>(base + kScratch.buf1)};
+  MrArgs a{};
+  a.digits = ka.digits;
+  a.pairs = reinterpret_cast<const MrPair*>(base + kScratch.mrpair);
+  a.npairs = 1;
+  for (uint32_t j = 0; j < levels; ++j) {
+    a.j = j;
+    a.cum = reinterpret_cast<const uint32_t*>(base + kScratch.mrcum) + 2 * j;
+    a.in = lv[j & 1];
+    a.out = lv[(j + 1) & 1];
+    HIP_TRY(launch_mr_level(a, cum[2 * j + 1], s));
+  }
+  uint32_t* d_res = reinterpret_cast<uint32_t*>(base + kScratch.mres);
+  uint32_t* d_len = reinterpret_cast<uint32_t*>(base + kScratch.mlen);
+  HIP_TRY(launch_mr_emit(a, lv[0], lv[1], d_res, d_len, s));
+  uint32_t len = 0;
+  HIP_TRY(hipMemcpyAsync(&len, d_len, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (len > k + 1) return MGCD_ERR_KERNEL;
+  *ng = len;
+  if (len > cap) return MGCD_ERR_CAPACITY;
+  if (len) {
+    HIP_TRY(hipMemcpyAsync(g, d_res, (size_t)len * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  return MGCD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -826,6 +1025,37 @@ int mgcd_gcd_limbs(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* 
     const int table = (int)ctx->moduli.size();
     if ((rc != MGCD_ERR_REDUX && rc != MGCD_ERR_RECOVERY) || !ctx->retry || ctx->force_nq || nq >= table) return rc;
     nq = std::min(table, ((nq + nq / 4 + 255) / 256) * 256);
+  }
+}
+
+int mgcd_gcd_limbs_noalloc(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint32_t* g,
+                           size_t cap, size_t* ng) {
+  if (!ctx || (nu && !u) || (nv && !v) || !ng) return MGCD_ERR_ARG;
+  *ng = 0;
+  HIP_TRY(hipSetDevice(ctx->device));
+  unsigned char* base = scratch_of(ctx->device);
+  if (!base) return MGCD_ERR_HIP;  // a context on this device took the address; none did
+  nu = trim(u, nu);
+  nv = trim(v, nv);
+  if (cmp_mag(u, nu, v, nv) < 0) {  // larger operand first (GmpCudaDevice-gcd.cu:61-62)
+    std::swap(u, v);
+    std::swap(nu, nv);
+  }
+  if (nv == 0) {
+    *ng = nu;
+    if (nu > cap) return MGCD_ERR_CAPACITY;
+    if (nu) std::memcpy(g, u, nu * 4);
+    return MGCD_OK;
+  }
+  if (nu + nv > kScratchMaxLimbs) return MGCD_ERR_SCRATCH;
+  uint32_t nq = (uint32_t)moduli_for(ctx, bit_length(u, nu));
+  if (nq > kScratchMaxModuli) return MGCD_ERR_SCRATCH;
+  // Retries as mgcd_gcd_limbs does, within the scratch limit.
+  const uint32_t top = std::min<uint32_t>((uint32_t)ctx->moduli.size(), kScratchMaxModuli);
+  for (;;) {
+    const int rc = gcd_single_noalloc(ctx, base, u, nu, v, nv, nq, g, cap, ng);
+    if ((rc != MGCD_ERR_REDUX && rc != MGCD_ERR_RECOVERY) || !ctx->retry || ctx->force_nq || nq >= top) return rc;
+    nq = std::min(top, ((nq + nq / 4 + 255) / 256) * 256);
   }
 }
 

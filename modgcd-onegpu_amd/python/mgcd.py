@@ -28,6 +28,7 @@ ERR_CAPACITY = 10
 ERR_NOMEM = 11
 ERR_TIMEOUT = 12
 ERR_BUSY = 13
+ERR_SCRATCH = 14
 
 # Every entry point declared in include/mgcd.h.
 EXPORTS = [
@@ -39,6 +40,7 @@ EXPORTS = [
     "mgcd_ctx_synchronize", "mgcd_batch_pair_stats", "mgcd_ctx_set_option", "mgcd_batch_launch_count",
     "mgcd_batch_launch_info", "mgcd_autotune", "mgcd_server_create", "mgcd_server_submit", "mgcd_server_wait",
     "mgcd_server_stats", "mgcd_server_destroy", "mgcd_mixed_radix_to_limbs", "mgcd_ctx_last_pair_stats",
+    "mgcd_gcd_limbs_noalloc",
 ]
 
 OPT_FORCE_MODULI = 1
@@ -89,6 +91,7 @@ def lib():
         L.mgcd_ctx_moduli.argtypes = [ctypes.c_void_p, P32, ctypes.c_size_t]
         L.mgcd_gcd_limbs.argtypes = [ctypes.c_void_p, P32, ctypes.c_size_t, P32, ctypes.c_size_t, P32,
                                      ctypes.c_size_t, PSZ]
+        L.mgcd_gcd_limbs_noalloc.argtypes = L.mgcd_gcd_limbs.argtypes
         L.mgcd_gcd_limbs_trace.argtypes = [ctypes.c_void_p, P32, ctypes.c_size_t, P32, ctypes.c_size_t, P32,
                                            ctypes.c_size_t, PSZ, PPair, ctypes.c_size_t, PSZ, PPair,
                                            ctypes.c_size_t, PSZ]
@@ -210,6 +213,16 @@ class Context:
         ng = ctypes.c_size_t()
         check(lib().mgcd_gcd_limbs(self.h, _p32(ua), len(ua), _p32(va), len(va), _p32(g), cap, ctypes.byref(ng)),
               "mgcd_gcd_limbs")
+        return limbs_to_int(g[:ng.value])
+
+    def gcd_noalloc(self, u, v):
+        """gcd(|u|, |v|) through mgcd_gcd_limbs_noalloc (no device allocation)."""
+        ua, va = int_to_limbs(u), int_to_limbs(v)
+        cap = max(len(ua), len(va)) + 1
+        g = np.zeros(cap, np.uint32)
+        ng = ctypes.c_size_t()
+        check(lib().mgcd_gcd_limbs_noalloc(self.h, _p32(ua), len(ua), _p32(va), len(va), _p32(g), cap,
+                                           ctypes.byref(ng)), "mgcd_gcd_limbs_noalloc")
         return limbs_to_int(g[:ng.value])
 
     def gcd_trace(self, u, v, dcap=None, tcap=None):
