@@ -28,10 +28,13 @@ import __graft_entry__ as ge  # noqa: E402
 # (= the FP64 vector peak 78.6 TFLOP/s counting an FMA as two flops).
 VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 
-# Algorithmic work per unit, in VALU issue slots of this build's gfx950 ISA
-# (instruction counts of the compiled loops; a quarter-rate v_rcp_f32 counts
-# 4).  Units are the algorithm's own: active modulus-iterations, not the
-# lanes the kernel keeps busy after their modulus retired.
+# Algorithmic work per unit, in VALU issue slots of gfx950 (a quarter-rate
+# v_rcp_f32 counts 4).  Units are the algorithm's own: active
+# modulus-iterations, not the lanes the kernel keeps busy after their modulus
+# retired.  FROZEN: the per-unit costs below were counted once from the
+# compiled loops (6 / 40 + 12.73 x 11 / B / 19 / 15) and are not recounted
+# when the kernel's instruction stream changes, so a leaner loop raises the
+# roofline fraction instead of shrinking its numerator.
 OPS_PER_RESIDUE_STEP = 6        # one Horner step per limb per modulus
 OPS_PER_VALUE = 40              # per active modulus-iteration and pair: t' = p'/(t - b) with the
                                 # batch-inversion share (4 mulmod), toSigned, key, select, reduce-scatter:
@@ -68,7 +71,7 @@ def main():
     ap.add_argument("--bits", type=int, default=100000)
     ap.add_argument("--gbits", type=int, default=10000)
     ap.add_argument("--L", type=int, default=32)
-    ap.add_argument("--per-launch-sample", type=int, default=32)
+    ap.add_argument("--per-launch-sample", type=int, default=128)
     ap.add_argument("--workgroup", type=int, default=0)
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--group", type=int, default=0)
@@ -164,7 +167,10 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        # Every core this process may run on, up to the 16-core host share a
+        # single-GPU box gives one GPU's jobs (its process guard counts threads).
+        affinity = len(os.sched_getaffinity(0))
+        threads = max(1, min(16, affinity))
         sample = min(args.pairs, 4096)
         limbs, u_off, nu, v_off, nv = m.pack_pairs(pairs[:sample])
         sec, gbits = O.time_mpz_gcd(limbs, u_off, nu, v_off, nv, threads)
@@ -174,9 +180,10 @@ def main():
             assert results[i] == O.mpz_gcd(*pairs[i]), f"pair {i}: gcd differs from mpz_gcd"
             exact_checked += 1
         cpu = {"value": sample / sec, "unit": "GCD/s", "cores": threads, "kind": "reference",
+               "affinity_cores": affinity, "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
                "sample": f"GMP {O.gmp_version()} mpz_gcd (the reference's CPU path, testmodgcd.cpp:261-263) on the "
-                         f"first {sample} pairs of the same workload, {threads} threads pulling pairs from a queue; "
-                         f"{sec:.2f} s wall"}
+                         f"first {sample} pairs of the same workload, {threads} threads pulling pairs from a queue "
+                         f"({affinity} cores in the affinity mask, {_cpu_model()}); {sec:.2f} s wall"}
         log(f"cpu baseline: {cpu['value']:.1f} GCD/s on {threads} threads")
 
     line = {
@@ -190,7 +197,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "exact integers held in f64 (FP32 reciprocal for quotient estimates)",
         "data": "synthetic",
         "config": {"workload": f"batched {args.pairs} x {args.bits}-bit pairs, {args.gbits}-bit shared factor, "
                                f"{args.L}-bit moduli", "pairs": args.pairs, "bits": args.bits,

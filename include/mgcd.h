@@ -37,7 +37,8 @@ typedef enum {
   MGCD_ERR_NOMEM = 11,             /* host or device allocation failed */
   MGCD_ERR_TIMEOUT = 12,           /* a bounded in-kernel wait expired */
   MGCD_ERR_BUSY = 13,              /* server: admission budget exhausted */
-  MGCD_ERR_SCRATCH = 14            /* input beyond the limit of mgcd_gcd_limbs_noalloc */
+  MGCD_ERR_SCRATCH = 14,           /* input beyond the limit of mgcd_gcd_limbs_noalloc */
+  MGCD_ERR_PENDING = 15            /* server: the wait expired before the request finished */
 } mgcd_status;
 
 /* One signed mixed-radix digit: the pair_t the reference kernel hands back
@@ -118,7 +119,14 @@ enum {
    * gives up and the pair reports MGCD_ERR_TIMEOUT: default 2^24, at least
    * 1.  Every wait is bounded by it, so a small value makes the kernel end
    * early with timeouts instead of results (used to test that path). */
-  MGCD_OPT_SPIN_LIMIT = 9
+  MGCD_OPT_SPIN_LIMIT = 9,
+  /* 1: the persistent kernel is launched cooperatively where the runtime
+   * allows it (all of its workgroups resident); 0: a plain launch of the same
+   * co-resident grid (on an otherwise idle device its workgroups are all
+   * resident too; a straggler only delays the bounded waits).  Default: the
+   * environment variable MGCD_COOPERATIVE if set, else 1.  Profilers that
+   * fail on cooperative launches (rocprofv3's teardown) run with 0. */
+  MGCD_OPT_COOPERATIVE = 10
 };
 int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value);
 /* Signed mixed-radix digits (least significant first, the layout the kernel
@@ -139,6 +147,19 @@ double mgcd_ctx_last_kernel_seconds(const mgcd_ctx* ctx);
  * whose harness times the call only (testmodgcd.cpp:265-275). */
 int mgcd_ctx_last_pair_stats(const mgcd_ctx* ctx, uint32_t* nq, uint32_t* iters, uint32_t* digits, int* workgroup,
                              int* slots, int* parts);
+
+/* Phase sums of the instrumented build of this library (lib/libmgcd_phases.so,
+ * built with -DMGCD_PHASE_CLOCKS; the default build returns MGCD_ERR_ARG).
+ * Over every work item the gcd kernel ran since the last reset, as wave 0 of
+ * each workgroup sees them: out[0..8] shader-clock cycles spent in residues,
+ * elimination compute, the barrier after it, the cross-workgroup exchange,
+ * the per-pair bookkeeping, the barrier ending a round, the step from
+ * elimination to replay, replay, recovery; out[9] elimination rounds;
+ * out[10] work items; out[11] 100 MHz real-time ticks and out[12] shader
+ * cycles over whole items (their ratio gives the clock).  Copies min(n, 16)
+ * values; reset != 0 clears the sums afterwards.  No counterpart in the
+ * reference. */
+int mgcd_ctx_phase_clocks(mgcd_ctx* ctx, uint64_t* out, size_t n, int reset);
 
 /* One pair, one launch sequence: the path of GmpCudaDevice::gcd
  * (GmpCudaDevice-gcd.cu:57-132) without GMP.  u, v may come in either order
@@ -191,6 +212,12 @@ int mgcd_batch_timing(const mgcd_batch* b, double* device_s, double* host_s);
 /* Work done for pair i in the last run: N_Q, elimination iterations R and
  * recovery digits k (0 for pairs answered on the host). */
 int mgcd_batch_pair_stats(const mgcd_batch* b, size_t i, uint32_t* nq, uint32_t* iters, uint32_t* digits);
+/* The elimination pivots (p, b) pair i used in the last run, in order, as
+ * the kernel recorded them (the reference's collect points,
+ * GmpCudaDevice-getGcdKernel.cu:371, :400): copies min(cap, count) of them,
+ * *n receives the count.  MGCD_ERR_ARG for a pair answered on the host
+ * (v = 0) or before the first run. */
+int mgcd_batch_pivots(const mgcd_batch* b, size_t i, mgcd_pair_t* pivots, size_t cap, size_t* n);
 /* The tiling of the launch with the most pairs: threads per workgroup,
  * moduli slots per thread, pairs per group. */
 int mgcd_batch_tiling(const mgcd_batch* b, int* workgroup, int* slots, int* group);
@@ -228,10 +255,11 @@ int mgcd_autotune(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, c
  *   submit: copies the operands; MGCD_ERR_BUSY if the request alone
  *           exceeds the budget.
  *   wait:   blocks until ticket's gcd is known (timeout_s < 0: no limit;
- *           MGCD_ERR_TIMEOUT after timeout_s seconds otherwise), then
- *           returns its status, its limbs in g and the latency from submit
- *           to completion, and releases the ticket (kept on
- *           MGCD_ERR_CAPACITY).
+ *           after timeout_s seconds otherwise MGCD_ERR_PENDING, and the
+ *           ticket stays valid), then returns its status (MGCD_ERR_TIMEOUT
+ *           only when an in-kernel wait of the request's launch expired),
+ *           its limbs in g and the latency from submit to completion, and
+ *           releases the ticket (kept on MGCD_ERR_CAPACITY).
  *   stats:  completed requests, p50 and p99 latency, completions per
  *           second since the first submit, peak resident bytes. */
 int mgcd_server_create(mgcd_ctx* ctx, uint64_t budget_bytes, int slots, mgcd_server** out);

@@ -77,7 +77,14 @@ struct KernelArgs {
   PairOut* out;
   uint32_t spin_limit;         // polls before a barrier wait gives up
   uint32_t relay_min_g;        // groups of at least this many parts exchange through part 0
+  unsigned long long* clocks;  // kPhaseClocks phase sums (instrumented build), else unused
 };
+
+// Phase sums of the instrumented build (-DMGCD_PHASE_CLOCKS, lib/libmgcd_phases.so):
+// see mgcd_ctx_phase_clocks.
+constexpr int kPhaseClocks = 16;
+// True in the instrumented build.
+bool phase_clocks_built();
 
 // Kernel variants: threads per workgroup T, moduli slots per thread K,
 // pairs per group B, for the two arithmetic classes (L <= 22: FP32

@@ -81,6 +81,46 @@ __device__ __forceinline__ void set_prio(int p) {
   }
 }
 
+// Phase clocks of the instrumented build (-DMGCD_PHASE_CLOCKS): shader-clock
+// cycles (s_memtime) per phase of a work item, as wave 0 sees them, summed
+// into args.clocks at the end of each item; the 100 MHz s_memrealtime total
+// calibrates the clock.  Without the macro every call compiles to nothing.
+enum : int {
+  PH_RESIDUES = 0, PH_COMPUTE, PH_BARRIER, PH_EXCHANGE, PH_BOOK, PH_BARRIER2, PH_TAIL, PH_REPLAY, PH_RECOVERY,
+  PH_ROUNDS, PH_ITEMS, PH_REALTIME, PH_MEMTIME
+};
+static_assert(PH_MEMTIME + 1 <= kPhaseClocks, "phase clock slots");
+struct PhaseClock {
+#ifdef MGCD_PHASE_CLOCKS
+  uint64_t acc[PH_ROUNDS] = {};
+  uint64_t t = 0, t0 = 0, r0 = 0, rounds = 0;
+  __device__ void start() {
+    t = t0 = __builtin_amdgcn_s_memtime();
+    r0 = __builtin_amdgcn_s_memrealtime();
+  }
+  __device__ void lap(int p) {
+    const uint64_t n = __builtin_amdgcn_s_memtime();
+    acc[p] += n - t;
+    t = n;
+  }
+  __device__ void round() { ++rounds; }
+  __device__ void flush(unsigned long long* out, bool writer) {
+    const uint64_t n = __builtin_amdgcn_s_memtime(), r = __builtin_amdgcn_s_memrealtime();
+    if (!writer || !out) return;
+    for (int p = 0; p < PH_ROUNDS; ++p) atomicAdd(out + p, (unsigned long long)acc[p]);
+    atomicAdd(out + PH_ROUNDS, (unsigned long long)rounds);
+    atomicAdd(out + PH_ITEMS, 1ull);
+    atomicAdd(out + PH_REALTIME, (unsigned long long)(r - r0));
+    atomicAdd(out + PH_MEMTIME, (unsigned long long)(n - t0));
+  }
+#else
+  __device__ void start() {}
+  __device__ void lap(int) {}
+  __device__ void round() {}
+  __device__ void flush(unsigned long long*, bool) {}
+#endif
+};
+
 // __ffs(abs(x)) as at getGcdKernel.cu:391 (abs(INT32_MIN) stays INT32_MIN).
 __device__ __forceinline__ int ffs_abs(int32_t x) {
   const uint32_t a = x < 0 ? 0u - (uint32_t)x : (uint32_t)x;
@@ -470,6 +510,8 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   const int rows = (int)((cnt + T - 1) / T);
   const bool lead = C.g == 0;
   const int L = args.L;
+  PhaseClock clk;
+  clk.start();
 
   // ---- (a) residues of u and v, stored canonical for (c) -----------------
   // The residues of part p (PP pairs, 2 PP operands, at a time).
@@ -602,6 +644,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   }
 
   // ---- (b) elimination ----------------------------------------------------
+  clk.lap(PH_RESIDUES);
   // The moduli table of the group in LDS, over (a)'s staging tile.
   const bool lds_mods = nq <= Shared<T, K, B>::MODS;
   __syncthreads();
@@ -670,8 +713,11 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
 
   // Reduces the wave minima over the group; wave 0 lane b < B gets pair b's.
   auto reduce_keys = [&](double w) -> double {
+    clk.lap(PH_COMPUTE);
+    clk.round();
     if ((lane & ((1u << SH) - 1)) == 0) S.red[lane >> SH][wave] = w;
     __syncthreads();
+    clk.lap(PH_BARRIER);
     double x = KNONE;
     if (wave == 0) {
       // The reduction, the exchange and the bookkeeping after it are the
@@ -687,6 +733,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       }
       if (C.G > 1) x = grid_min<B>(x, C);
     }
+    clk.lap(PH_EXCHANGE);
     return x;
   };
 
@@ -731,7 +778,9 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       if (lane == 0) S.flag = any;
       if constexpr (B > 1) set_prio(0);
     }
+    clk.lap(PH_BOOK);
     __syncthreads();
+    clk.lap(PH_BARRIER2);
   }
 
   while (S.flag) {
@@ -899,7 +948,9 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       if (lane == 0) S.flag = any;
       if constexpr (B > 1) set_prio(0);
     }
+    clk.lap(PH_BOOK);
     __syncthreads();
+    clk.lap(PH_BARRIER2);
   }
 
   // Surviving moduli of each pair, as ballot words.
@@ -923,6 +974,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   __syncthreads();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
+  clk.lap(PH_TAIL);
   // ---- (c1) replay of the pivots on the survivors -------------------------
   // The survivors of every pair of the group form one compacted list
   // (pair-major; entry = pair << 11 | slot offset); each lane replays up to
@@ -1027,6 +1079,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
     __syncthreads();
   }
 
+  clk.lap(PH_REPLAY);
   // ---- (c2) recovery, RB pairs at a time in lockstep (:404-433) -----------
   // u = a/s per surviving modulus; a lane inverts s only for a digit it owns.
   // Chunk c holds pairs c*RB + j; wave 0 lane c*RB + j keeps pair c*RB + j's
@@ -1214,6 +1267,8 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
     o.pad = 0;
     args.out[Gd.first + lane] = o;
   }
+  clk.lap(PH_RECOVERY);
+  clk.flush(args.clocks, tid == 0);
 }
 
 // Next work item for this workgroup (thread 0): queue 0, then the queue of
@@ -1358,15 +1413,29 @@ __global__ void certify_rcp_kernel(unsigned long long* fail, unsigned long long*
 // ---------------------------------------------------------------------------
 // Variant table.
 #define MGCD_VARIANT(T, K, B, A, W) {T, K, B, W, reinterpret_cast<const void*>(&gcd_kernel<T, K, B, A>)}
+#ifdef MGCD_PHASE_CLOCKS
+// The instrumented build: the one-pair tilings and the default batch tiling.
+#define MGCD_VARIANTS(A, W) \
+  MGCD_VARIANT(256, 1, 1, A, W), MGCD_VARIANT(1024, 1, 1, A, W), MGCD_VARIANT(512, 2, 16, A, W)
+#else
 #define MGCD_VARIANTS(A, W)                                                                              \
   MGCD_VARIANT(256, 1, 1, A, W), MGCD_VARIANT(256, 2, 1, A, W), MGCD_VARIANT(256, 4, 1, A, W),           \
       MGCD_VARIANT(256, 8, 1, A, W), MGCD_VARIANT(1024, 1, 1, A, W), MGCD_VARIANT(1024, 2, 1, A, W),     \
       MGCD_VARIANT(256, 1, 8, A, W), MGCD_VARIANT(256, 2, 8, A, W), MGCD_VARIANT(256, 1, 16, A, W),      \
       MGCD_VARIANT(256, 2, 16, A, W), MGCD_VARIANT(256, 1, 32, A, W), MGCD_VARIANT(128, 1, 16, A, W),    \
       MGCD_VARIANT(512, 2, 16, A, W), MGCD_VARIANT(512, 1, 16, A, W)
+#endif
 static const KernelVariant kVariants[] = {MGCD_VARIANTS(Arith32, 1), MGCD_VARIANTS(Arith22, 0)};
 #undef MGCD_VARIANTS
 #undef MGCD_VARIANT
+
+bool phase_clocks_built() {
+#ifdef MGCD_PHASE_CLOCKS
+  return true;
+#else
+  return false;
+#endif
+}
 
 int kernel_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
 const KernelVariant& kernel_variant(int i) { return kVariants[i]; }

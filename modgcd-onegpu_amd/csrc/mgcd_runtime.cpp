@@ -6,7 +6,7 @@
 // persistent kernel cooperatively so that all of its workgroups are
 // resident (the cross-workgroup waits need that), and converts the signed
 // mixed-radix digits back to positional limbs (the host Horner of
-// GmpCudaDevice-gcd.cu:243-256, without GMP).
+// GmpCudaDevice-gcd.cu:116-129, without GMP).
 #include <dlfcn.h>
 
 #include <algorithm>
@@ -50,6 +50,7 @@ const char* kMessages[] = {
     "A device-side wait timed out.",
     "Admission budget exhausted.",
     "Input exceeds the no-allocation scratch limit.",
+    "Request still pending.",
 };
 
 thread_local std::string g_last_error;
@@ -158,8 +159,10 @@ uint32_t ceil_log2(uint32_t k) {
 // |G| of every (dig_off, k) listed, from the digits in device memory at
 // d_digits, by the product tree of mgcd_convert.hip on stream s; waits for
 // the stream.  out[i] receives pair i's limbs.  Returns an MGCD_ status.
+// retired: as in DevBuf::ensure (callers whose device runs other streams' kernels).
 int mr_convert(hipStream_t s, const mgcd_pair_t* d_digits, const std::vector<uint64_t>& dig_off,
-               const std::vector<uint32_t>& ks, MrBufs& mb, std::vector<std::vector<uint32_t>*>& out) {
+               const std::vector<uint32_t>& ks, MrBufs& mb, std::vector<std::vector<uint32_t>*>& out,
+               std::vector<void*>* retired = nullptr) {
   const uint32_t n = (uint32_t)ks.size();
   if (n == 0) return MGCD_OK;
   mb.h_pairs.resize(n);
@@ -187,9 +190,10 @@ int mr_convert(hipStream_t s, const mgcd_pair_t* d_digits, const std::vector<uin
       c[i + 1] = c[i] + t;
     }
   }
-  if (mb.pairs.ensure(n * sizeof(MrPair)) != hipSuccess || mb.cum.ensure(mb.h_cum.size() * 4) != hipSuccess ||
-      mb.buf0.ensure(buf * 4) != hipSuccess || mb.buf1.ensure(buf * 4) != hipSuccess ||
-      mb.res.ensure(res * 4) != hipSuccess || mb.lens.ensure(n * 4) != hipSuccess)
+  if (mb.pairs.ensure(n * sizeof(MrPair), retired) != hipSuccess ||
+      mb.cum.ensure(mb.h_cum.size() * 4, retired) != hipSuccess || mb.buf0.ensure(buf * 4, retired) != hipSuccess ||
+      mb.buf1.ensure(buf * 4, retired) != hipSuccess || mb.res.ensure(res * 4, retired) != hipSuccess ||
+      mb.lens.ensure(n * 4, retired) != hipSuccess)
     return MGCD_ERR_NOMEM;
   HIP_TRY(hipMemcpyAsync(mb.pairs.p, mb.h_pairs.data(), n * sizeof(MrPair), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(mb.cum.p, mb.h_cum.data(), mb.h_cum.size() * 4, hipMemcpyHostToDevice, s));
@@ -219,7 +223,7 @@ int mr_convert(hipStream_t s, const mgcd_pair_t* d_digits, const std::vector<uin
 }
 
 // Signed mixed-radix digits (least significant first) -> |G| as limbs:
-// G = g_k; G = G * m_i + g_i for i = k-1 .. 1 (GmpCudaDevice-gcd.cu:243-256).
+// G = g_k; G = G * m_i + g_i for i = k-1 .. 1 (GmpCudaDevice-gcd.cu:116-129).
 // When G != 0, |G * m_i| >= m_i > |g_i|, so the sign never flips mid-way.
 void digits_to_limbs(const mgcd_pair_t* d, size_t k, std::vector<uint32_t>& mag) {
   mag.clear();
@@ -383,8 +387,10 @@ struct mgcd_ctx {
   uint32_t relay_min_g = 48;  // MGCD_OPT_RELAY_PARTS
   bool device_convert = true;  // MGCD_OPT_DEVICE_CONVERT
   uint32_t spin_limit = kSpinLimit;  // MGCD_OPT_SPIN_LIMIT
+  bool cooperative = true;     // MGCD_OPT_COOPERATIVE (default: $MGCD_COOPERATIVE, else 1)
   std::map<std::string, int> tuned;  // size class -> kernel variant
   MrBufs mr;                   // the one-pair path's conversion buffers
+  DevBuf clocks;               // phase sums (instrumented build only)
 };
 
 // A prepared launch over device-resident pairs.
@@ -408,11 +414,14 @@ namespace {
 // G parts of a group consecutive, so groups spanning several workgroups are
 // popped first.  pd is reordered into group order; order[j] is the caller's
 // index of kernel pair j.
+// grid_cap (0: none) bounds the grid below the variant's co-resident limit
+// (a serving slot's share of the device); the queue plan is made for it.
 int plan_items(mgcd_ctx* ctx, int vi, std::vector<PairDesc>& pd, std::vector<uint32_t>& order,
-               std::vector<GroupDesc>& groups, std::vector<WorkItem>& items, LaunchSet& ls) {
+               std::vector<GroupDesc>& groups, std::vector<WorkItem>& items, LaunchSet& ls, uint32_t grid_cap = 0) {
   const KernelVariant& kv = kernel_variant(vi);
   const uint32_t per_wg = (uint32_t)(kv.T * kv.K);
-  const uint32_t maxres = (uint32_t)ctx->max_resident[vi];
+  const uint32_t maxres =
+      grid_cap ? std::min<uint32_t>(grid_cap, (uint32_t)ctx->max_resident[vi]) : (uint32_t)ctx->max_resident[vi];
   if (maxres == 0) return MGCD_ERR_TOO_MANY_THREADS;
   order.resize(pd.size());
   for (uint32_t i = 0; i < pd.size(); ++i) order[i] = i;
@@ -527,7 +536,7 @@ int launch(mgcd_ctx* ctx, const LaunchSet& ls, const KernelArgs& ka) {
   std::memcpy(args.qoff, ls.qoff, sizeof args.qoff);
   void* params[] = {&args};
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-  HIP_TRY(launch_resident(kv.fn, ls.grid, kv.T, params, ctx->stream, true));
+  HIP_TRY(launch_resident(kv.fn, ls.grid, kv.T, params, ctx->stream, ctx->cooperative));
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
   return MGCD_OK;
 }
@@ -616,6 +625,7 @@ int mgcd_ctx_create(mgcd_ctx** out, int device, int L, const char* table_path) {
   ctx->device = device;
   ctx->L = L;
   ctx->wide = L > 22 ? 1 : 0;
+  if (const char* co = std::getenv("MGCD_COOPERATIVE")) ctx->cooperative = std::atoi(co) != 0;
   ctx->moduli = std::move(mod);
   hipDeviceProp_t props;
   hipError_t e = hipGetDeviceProperties(&props, device);
@@ -653,6 +663,12 @@ int mgcd_ctx_create(mgcd_ctx** out, int device, int L, const char* table_path) {
              hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
     fail(MGCD_ERR_HIP);
   }
+  if (rc == MGCD_OK && phase_clocks_built()) {
+    if (ctx->clocks.ensure(kPhaseClocks * 8) != hipSuccess)
+      fail(MGCD_ERR_NOMEM);
+    else if (hipMemset(ctx->clocks.p, 0, kPhaseClocks * 8) != hipSuccess)
+      fail(MGCD_ERR_HIP);
+  }
   if (rc == MGCD_OK) {
     ctx->max_resident.assign(kernel_variant_count(), 0);
     for (int i = 0; i < kernel_variant_count(); ++i) {
@@ -675,7 +691,7 @@ int mgcd_ctx_destroy(mgcd_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (DevBuf* b : {&ctx->d_mod_u32, &ctx->d_mod_d, &ctx->d_mod_inv, &ctx->limbs, &ctx->pairs, &ctx->groups, &ctx->items,
-                    &ctx->queue, &ctx->bar, &ctx->digits, &ctx->trace, &ctx->res, &ctx->out})
+                    &ctx->queue, &ctx->bar, &ctx->digits, &ctx->trace, &ctx->res, &ctx->out, &ctx->clocks})
     b->release();
   ctx->mr.release();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -733,6 +749,9 @@ int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value) {
       if (value < 1 || value > (int64_t)1 << 31) return MGCD_ERR_ARG;
       ctx->spin_limit = (uint32_t)value;
       return MGCD_OK;
+    case MGCD_OPT_COOPERATIVE:
+      ctx->cooperative = value != 0;
+      return MGCD_OK;
     default:
       return MGCD_ERR_ARG;
   }
@@ -748,6 +767,13 @@ namespace {
 int moduli_for(const mgcd_ctx* ctx, uint64_t bits) {
   if (ctx->force_nq) return (int)ctx->force_nq;
   return mgcd_num_moduli_needed(std::max<uint64_t>(bits, 2), ctx->L);
+}
+
+// The last-launch figures of a one-pair call that may not launch (v = 0).
+void clear_last_pair(mgcd_ctx* ctx) {
+  ctx->last_kernel_s = 0.0;
+  ctx->last_nq = ctx->last_iters = ctx->last_digits = 0;
+  ctx->last_T = ctx->last_K = ctx->last_G = 0;
 }
 
 // One pair, u taken as given, with N_Q = nq moduli.
@@ -808,6 +834,7 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
   ka.out = ctx->out.as<PairOut>();
   ka.spin_limit = ctx->spin_limit;
   ka.relay_min_g = ctx->relay_min_g;
+  ka.clocks = ctx->clocks.as<unsigned long long>();
   rc = launch(ctx, ls, ka);
   if (rc != MGCD_OK) return rc;
   PairOut po;
@@ -920,6 +947,12 @@ int gcd_single_noalloc(mgcd_ctx* ctx, unsigned char* base, const uint32_t* u, si
   HIP_TRY(hipStreamSynchronize(s));
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_kernel_s = ms * 1e-3;
+  ctx->last_nq = nq;
+  ctx->last_iters = po.ntrace;
+  ctx->last_digits = po.ndig;
+  ctx->last_T = kv.T;
+  ctx->last_K = kv.K;
+  ctx->last_G = (int)G;
   if (po.status != KST_OK) return kst_to_status(po.status);
   if (po.ndig > nq) return MGCD_ERR_KERNEL;
   const uint32_t k = po.ndig;
@@ -980,6 +1013,7 @@ int mgcd_gcd_limbs_trace(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint
   nv = trim(v, nv);
   const uint64_t bits = bit_length(u, nu);
   if (bits == 0) return MGCD_ERR_ARG;
+  clear_last_pair(ctx);
   return gcd_single(ctx, u, nu, v, nv, moduli_for(ctx, bits), g, cap, ng, digits, dcap, nd, pivots, tcap, nt);
 }
 
@@ -1005,6 +1039,7 @@ int mgcd_gcd_limbs(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* 
                    size_t cap, size_t* ng) {
   if (!ctx || (nu && !u) || (nv && !v) || !ng) return MGCD_ERR_ARG;
   HIP_TRY(hipSetDevice(ctx->device));
+  clear_last_pair(ctx);
   nu = trim(u, nu);
   nv = trim(v, nv);
   if (cmp_mag(u, nu, v, nv) < 0) {  // larger operand first (GmpCudaDevice-gcd.cu:61-62)
@@ -1035,6 +1070,7 @@ int mgcd_gcd_limbs_noalloc(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const ui
   HIP_TRY(hipSetDevice(ctx->device));
   unsigned char* base = scratch_of(ctx->device);
   if (!base) return MGCD_ERR_HIP;  // a context on this device took the address; none did
+  clear_last_pair(ctx);
   nu = trim(u, nu);
   nv = trim(v, nv);
   if (cmp_mag(u, nu, v, nv) < 0) {  // larger operand first (GmpCudaDevice-gcd.cu:61-62)
@@ -1060,6 +1096,21 @@ int mgcd_gcd_limbs_noalloc(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const ui
 }
 
 double mgcd_ctx_last_kernel_seconds(const mgcd_ctx* ctx) { return ctx ? ctx->last_kernel_s : 0.0; }
+
+int mgcd_ctx_phase_clocks(mgcd_ctx* ctx, uint64_t* out, size_t n, int reset) {
+  if (!ctx || (n && !out)) return MGCD_ERR_ARG;
+  if (!phase_clocks_built() || !ctx->clocks.p) {
+    g_last_error = "not the instrumented build (lib/libmgcd_phases.so)";
+    return MGCD_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  uint64_t h[kPhaseClocks];
+  HIP_TRY(hipMemcpy(h, ctx->clocks.p, sizeof h, hipMemcpyDeviceToHost));
+  std::memcpy(out, h, std::min<size_t>(n, kPhaseClocks) * 8);
+  if (reset) HIP_TRY(hipMemset(ctx->clocks.p, 0, sizeof h));
+  return MGCD_OK;
+}
 
 int mgcd_ctx_last_pair_stats(const mgcd_ctx* ctx, uint32_t* nq, uint32_t* iters, uint32_t* digits, int* workgroup,
                              int* slots, int* parts) {
@@ -1170,6 +1221,7 @@ KernelArgs batch_args(const mgcd_batch* b, const BatchLaunch& l) {
   ka.out = b->out.as<PairOut>();
   ka.spin_limit = ctx->spin_limit;
   ka.relay_min_g = ctx->relay_min_g;
+  ka.clocks = ctx->clocks.as<unsigned long long>();
   ka.nqueues = l.ls.nqueues;
   std::memcpy(ka.qoff, l.ls.qoff, sizeof ka.qoff);
   return ka;
@@ -1475,7 +1527,7 @@ int mgcd_batch_run(mgcd_batch* b) {
       HIP_TRY(hipEventRecord(ctx->ev0, s));
       for (const BatchLaunch& l : b->launches) {
         HIP_TRY(hipEventRecord(l.e0, s));
-        const int rc = enqueue_launch(b, l, true);
+        const int rc = enqueue_launch(b, l, ctx->cooperative);
         if (rc != MGCD_OK) return rc;
         HIP_TRY(hipEventRecord(l.e1, s));
       }
@@ -1599,6 +1651,22 @@ int mgcd_batch_timing(const mgcd_batch* b, double* device_s, double* host_s) {
   if (!b) return MGCD_ERR_ARG;
   if (device_s) *device_s = b->device_s;
   if (host_s) *host_s = b->host_s;
+  return MGCD_OK;
+}
+
+int mgcd_batch_pivots(const mgcd_batch* b, size_t i, mgcd_pair_t* pivots, size_t cap, size_t* n) {
+  if (!b || i >= b->npairs || !n || (cap && !pivots)) return MGCD_ERR_ARG;
+  *n = 0;
+  if (!b->ran || b->kidx[i] < 0) return MGCD_ERR_ARG;
+  const PairDesc& p = b->pd[b->kidx[i]];
+  const size_t nt = std::min<size_t>(b->h_out[b->kidx[i]].ntrace, p.trace_cap);
+  *n = nt;
+  const size_t c = std::min(nt, cap);
+  if (c) {
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipMemcpy(pivots, b->trace.as<mgcd_pair_t>() + p.trace_off, c * sizeof(mgcd_pair_t),
+                      hipMemcpyDeviceToHost));
+  }
   return MGCD_OK;
 }
 
@@ -1772,7 +1840,8 @@ struct ServerSlot {
   DevBuf limbs, pairs, groups, items, queue, bar, digits, trace, res, out;
   // Pinned staging, so that neither direction of copy blocks the worker
   // while other slots' kernels run.
-  PinnedBuf h_in, h_out, h_digits;
+  PinnedBuf h_in, h_out;
+  MrBufs mr;  // conversion of the digits on the device
   // Buffers outgrown while serving, freed with the server: hipFree and
   // hipHostFree wait for every stream, which would stall this slot behind
   // the other slots' kernels.
@@ -1862,25 +1931,18 @@ int server_launch(mgcd_server* s, ServerSlot& sl) {
     }
   }
   sl.variant = vi;
-  int rc = plan_items(ctx, vi, sl.pd, sl.order, groups, items, sl.ls);
+  // Every slot's grid fits the device at the same time as the others': the
+  // plan (grid, queues) is made for this slot's share.
+  const uint32_t share = (uint32_t)(ctx->max_resident[vi] / (int)s->slots.size());
+  if (share == 0) return MGCD_ERR_TOO_MANY_THREADS;
+  int rc = plan_items(ctx, vi, sl.pd, sl.order, groups, items, sl.ls, share);
   if (rc != MGCD_OK) return rc;
-  // Every slot's grid fits the device at the same time as the others'.
-  sl.ls.grid = std::min(sl.ls.grid, (uint32_t)(ctx->max_resident[vi] / (int)s->slots.size()));
-  uint32_t maxG = 0;
-  for (const GroupDesc& g : groups) maxG = std::max(maxG, g.G);
-  if (maxG > sl.ls.grid) return MGCD_ERR_TOO_MANY_THREADS;
-  if (sl.ls.nqueues > 1 && maxG > sl.ls.grid / 2) {  // keep the queue plan's no-stall condition
-    sl.ls.nqueues = 1;
-    sl.ls.qoff[0] = 0;
-    sl.ls.qoff[1] = (uint32_t)items.size();
-  }
   const size_t np = sl.pd.size();
   const size_t in_bytes = packed.size() * 4 + np * sizeof(PairDesc) + groups.size() * sizeof(GroupDesc) +
                           items.size() * sizeof(WorkItem) + 64;
   std::vector<void*>* rh = &sl.retired_host;
   std::vector<void*>* rd = &sl.retired_dev;
-  if (sl.h_in.ensure(in_bytes, rh) != hipSuccess || sl.h_out.ensure(np * sizeof(PairOut), rh) != hipSuccess ||
-      sl.h_digits.ensure(sl.ls.ndigits * 8 + 8, rh) != hipSuccess)
+  if (sl.h_in.ensure(in_bytes, rh) != hipSuccess || sl.h_out.ensure(np * sizeof(PairOut), rh) != hipSuccess)
     return MGCD_ERR_NOMEM;
   if (sl.limbs.ensure(packed.size() * 4 + 4, rd) != hipSuccess ||
       sl.pairs.ensure(np * sizeof(PairDesc), rd) != hipSuccess ||
@@ -1932,7 +1994,6 @@ int server_launch(mgcd_server* s, ServerSlot& sl) {
   void* params[] = {&ka};
   HIP_TRY(launch_resident(kv.fn, sl.ls.grid, kv.T, params, st, false));
   HIP_TRY(hipMemcpyAsync(sl.h_out.p, sl.out.p, np * sizeof(PairOut), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(sl.h_digits.p, sl.digits.p, sl.ls.ndigits * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipEventRecord(sl.done_ev, st));
   return MGCD_OK;
 }
@@ -1942,15 +2003,32 @@ int server_launch(mgcd_server* s, ServerSlot& sl) {
 // one-pair path with the retry policy.
 void server_finish(mgcd_server* s, ServerSlot& sl, int launch_rc) {
   mgcd_ctx* ctx = s->ctx;
+  // The digits of the finished pairs become |G| on the device (the product
+  // tree batches use); only |G| comes back.
+  std::vector<int> sts(sl.pd.size(), launch_rc);
+  std::vector<uint64_t> offs;
+  std::vector<uint32_t> ks;
+  std::vector<std::vector<uint32_t>*> outs;
+  for (size_t j = 0; j < sl.pd.size() && launch_rc == MGCD_OK; ++j) {
+    const PairOut& o = sl.h_out.as<PairOut>()[j];
+    ServerReq* r = sl.reqs[sl.order[j]];
+    sts[j] = o.status != KST_OK ? kst_to_status(o.status) : (o.ndig > sl.pd[j].nq ? MGCD_ERR_KERNEL : MGCD_OK);
+    r->result.clear();
+    if (sts[j] == MGCD_OK && o.ndig > 0) {
+      offs.push_back(sl.pd[j].dig_off);
+      ks.push_back(o.ndig);
+      outs.push_back(&r->result);
+    }
+  }
+  if (!ks.empty()) {
+    const int crc = mr_convert(sl.stream, sl.digits.as<mgcd_pair_t>(), offs, ks, sl.mr, outs, &sl.retired_dev);
+    if (crc != MGCD_OK)
+      for (int& st : sts)
+        if (st == MGCD_OK) st = crc;
+  }
   for (size_t j = 0; j < sl.pd.size(); ++j) {
     ServerReq* r = sl.reqs[sl.order[j]];
-    const PairDesc& p = sl.pd[j];
-    int st = launch_rc;
-    if (st == MGCD_OK) {
-      const PairOut& o = sl.h_out.as<PairOut>()[j];
-      st = o.status != KST_OK ? kst_to_status(o.status) : (o.ndig > p.nq ? MGCD_ERR_KERNEL : MGCD_OK);
-      if (st == MGCD_OK) digits_to_limbs(sl.h_digits.as<mgcd_pair_t>() + p.dig_off, o.ndig, r->result);
-    }
+    int st = sts[j];
     if (st == MGCD_ERR_REDUX || st == MGCD_ERR_RECOVERY || st == MGCD_ERR_TIMEOUT) {
       std::vector<uint32_t> g(r->v.size() + 1);
       size_t ng = 0;
@@ -2145,7 +2223,7 @@ int mgcd_server_wait(mgcd_server* s, uint64_t ticket, double timeout_s, uint32_t
   if (timeout_s < 0)
     s->cv.wait(lk, pred);
   else if (!s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
-    return MGCD_ERR_TIMEOUT;
+    return MGCD_ERR_PENDING;  // ticket kept: ask again
   if (latency_s) *latency_s = r->latency;
   *ng = r->result.size();
   const int st = r->status;
@@ -2188,7 +2266,8 @@ int mgcd_server_destroy(mgcd_server* s) {
     for (DevBuf* d : {&sl.limbs, &sl.pairs, &sl.groups, &sl.items, &sl.queue, &sl.bar, &sl.digits, &sl.trace, &sl.res,
                       &sl.out})
       d->release();
-    for (PinnedBuf* h : {&sl.h_in, &sl.h_out, &sl.h_digits}) h->release();
+    for (PinnedBuf* h : {&sl.h_in, &sl.h_out}) h->release();
+    sl.mr.release();
     for (void* p : sl.retired_dev) (void)hipFree(p);
     for (void* p : sl.retired_host) (void)hipHostFree(p);
     if (sl.done_ev) (void)hipEventDestroy(sl.done_ev);

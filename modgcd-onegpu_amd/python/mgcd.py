@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "lib", "libmgcd.so")
+# MGCD_LIB selects another build of the library (the instrumented libmgcd_phases.so).
+LIB_PATH = os.environ.get("MGCD_LIB") or os.path.join(PKG_DIR, "lib", "libmgcd.so")
 GMPCUDA_PATH = os.path.join(PKG_DIR, "lib", "libgmpcuda.so")
 DATA_DIR = os.path.join(PKG_DIR, "data")
 
@@ -29,6 +30,7 @@ ERR_NOMEM = 11
 ERR_TIMEOUT = 12
 ERR_BUSY = 13
 ERR_SCRATCH = 14
+ERR_PENDING = 15
 
 # Every entry point declared in include/mgcd.h.
 EXPORTS = [
@@ -40,7 +42,7 @@ EXPORTS = [
     "mgcd_ctx_synchronize", "mgcd_batch_pair_stats", "mgcd_ctx_set_option", "mgcd_batch_launch_count",
     "mgcd_batch_launch_info", "mgcd_autotune", "mgcd_server_create", "mgcd_server_submit", "mgcd_server_wait",
     "mgcd_server_stats", "mgcd_server_destroy", "mgcd_mixed_radix_to_limbs", "mgcd_ctx_last_pair_stats",
-    "mgcd_gcd_limbs_noalloc",
+    "mgcd_gcd_limbs_noalloc", "mgcd_batch_pivots", "mgcd_ctx_phase_clocks",
 ]
 
 OPT_FORCE_MODULI = 1
@@ -52,6 +54,7 @@ OPT_BATCH_GRAPH = 6
 OPT_RELAY_PARTS = 7
 OPT_DEVICE_CONVERT = 8
 OPT_SPIN_LIMIT = 9
+OPT_COOPERATIVE = 10
 
 
 class Pair(ctypes.Structure):
@@ -113,6 +116,8 @@ def lib():
         L.mgcd_ctx_last_pair_stats.argtypes = [ctypes.c_void_p, P32, P32, P32, ctypes.POINTER(ctypes.c_int),
                                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.mgcd_ctx_set_option.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]
+        L.mgcd_ctx_phase_clocks.argtypes = [ctypes.c_void_p, P64, ctypes.c_size_t, ctypes.c_int]
+        L.mgcd_batch_pivots.argtypes = [ctypes.c_void_p, ctypes.c_size_t, PPair, ctypes.c_size_t, PSZ]
         L.mgcd_batch_pair_stats.argtypes = [ctypes.c_void_p, ctypes.c_size_t, P32, P32, P32]
         L.mgcd_batch_launch_count.argtypes = [ctypes.c_void_p, PSZ]
         L.mgcd_batch_launch_info.argtypes = [ctypes.c_void_p, ctypes.c_size_t, PI, PI, PI, PSZ, P32, P32,
@@ -268,6 +273,19 @@ class Context:
     def last_kernel_seconds(self):
         return lib().mgcd_ctx_last_kernel_seconds(self.h)
 
+    PHASES = ["residues", "elim_compute", "elim_barrier", "elim_exchange", "elim_bookkeeping", "elim_barrier2",
+              "elim_tail", "replay", "recovery"]
+
+    def phase_clocks(self, reset=True):
+        """Phase sums of the instrumented build (MGCD_LIB=...libmgcd_phases.so): dict of shader
+        cycles per phase plus rounds, items, realtime ticks (100 MHz) and memtime cycles."""
+        out = np.zeros(16, np.uint64)
+        check(lib().mgcd_ctx_phase_clocks(self.h, out.ctypes.data_as(P64), 16, 1 if reset else 0),
+              "mgcd_ctx_phase_clocks")
+        d = {name: int(out[i]) for i, name in enumerate(self.PHASES)}
+        d.update(rounds=int(out[9]), items=int(out[10]), realtime_ticks=int(out[11]), memtime_cycles=int(out[12]))
+        return d
+
     def last_pair_stats(self):
         """dict(nq, iters, digits, tiling=(T, K), parts) of the last one-pair launch."""
         q, r, k = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
@@ -354,6 +372,14 @@ class Batch:
         check(lib().mgcd_batch_pair_stats(self.h, i, ctypes.byref(q), ctypes.byref(r), ctypes.byref(k)))
         return q.value, r.value, k.value
 
+    def pivots(self, i):
+        """Pair i's elimination pivots [(p, b), ...] from the last run."""
+        n = ctypes.c_size_t()
+        check(lib().mgcd_batch_pivots(self.h, i, None, 0, ctypes.byref(n)), f"mgcd_batch_pivots[{i}]")
+        arr = (Pair * max(n.value, 1))()
+        check(lib().mgcd_batch_pivots(self.h, i, arr, n.value, ctypes.byref(n)), f"mgcd_batch_pivots[{i}]")
+        return [(arr[j].modulus, arr[j].value) for j in range(n.value)]
+
     def tiling(self):
         t, k, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib().mgcd_batch_tiling(self.h, ctypes.byref(t), ctypes.byref(k), ctypes.byref(g)))
@@ -406,11 +432,15 @@ class Server:
         return t.value
 
     def wait(self, ticket, timeout=-1.0):
-        cap = self._cap.pop(ticket)
+        """(gcd, latency seconds); raises MgcdError with status ERR_PENDING when
+        the timeout expires first (the ticket stays valid: wait again)."""
+        cap = self._cap[ticket]
         g = np.zeros(cap, np.uint32)
         ng, lat = ctypes.c_size_t(), ctypes.c_double()
-        check(lib().mgcd_server_wait(self.h, ticket, timeout, _p32(g), cap, ctypes.byref(ng), ctypes.byref(lat)),
-              "mgcd_server_wait")
+        st = lib().mgcd_server_wait(self.h, ticket, timeout, _p32(g), cap, ctypes.byref(ng), ctypes.byref(lat))
+        if st not in (ERR_PENDING, ERR_CAPACITY):
+            del self._cap[ticket]
+        check(st, "mgcd_server_wait")
         return limbs_to_int(g[:ng.value]), lat.value
 
     def stats(self):

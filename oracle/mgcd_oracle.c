@@ -324,7 +324,7 @@ int oracle_gcd_kernel_exact(const uint32_t* u, size_t nu, const uint32_t* v, siz
 }
 
 /* ---------------------------------------------------------------------------
- * Host side (GmpCudaDevice-gcd.cu:243-256): Horner over the signed digits,
+ * Host side (GmpCudaDevice-gcd.cu:116-129): Horner over the signed digits,
  * most significant first, then |G|.  Output as 32-bit limbs, least
  * significant first.
  */

@@ -116,9 +116,12 @@ def test_small_and_ragged_sizes(ctx_for, mgcd, L):
         assert ctx.gcd(u, v) == math.gcd(u, v), (u.bit_length(), v.bit_length())
 
 
-@pytest.mark.parametrize("L", [32, 27, 22])
-def test_single_large_1M(ctx_for, oracle, wl, L):
-    u, v = wl.single_large(0)
+# The ~1M-bit one-pair configuration (g 100k-bit, a, b 900k-bit): seeds 0-9 at
+# L = 32, two seeds each at L = 27 and L = 22, every result against mpz_gcd
+# (the reference's check, testmodgcd.cpp:283-295).
+@pytest.mark.parametrize("L,seed", [(32, s) for s in range(10)] + [(27, 0), (27, 1), (22, 0), (22, 1)])
+def test_single_large_1M(ctx_for, oracle, wl, L, seed):
+    u, v = wl.single_large(seed)
     g = ctx_for(L).gcd(u, v)
     assert g == oracle.mpz_gcd(u, v)
     assert g.bit_length() >= 100000
@@ -192,7 +195,7 @@ def test_batch_tilings_agree(ctx_for, mgcd, wl, T, K, B):
 
 
 @pytest.mark.parametrize("T, K, B", [(512, 2, 16), (256, 1, 16)])
-def test_batch_sentinel_states_match_single(ctx_for, mgcd, wl, T, K, B):
+def test_batch_sentinel_states_match_single(ctx_for, mgcd, oracle, wl, T, K, B):
     # Grouped pairs whose operands table moduli divide (t starting at
     # infinity, 0/0 or 0 on those moduli), beside coprime pairs that finish
     # at infinity early and stay in the group: each pair's iteration and
@@ -212,16 +215,22 @@ def test_batch_sentinel_states_match_single(ctx_for, mgcd, wl, T, K, B):
         st, g, d, p = ctx.gcd_trace(max(u, v), min(u, v))
         nq, R, k = b.pair_stats(i)
         assert (R, k) == (len(p), len(d)) and b.result(i) == g == math.gcd(u, v), i
+        # pivot by pivot: the grouped pair's recorded pivots are the oracle's
+        ost, od, ot = oracle.gcd_kernel(max(u, v), min(u, v), mods[:nq], 32)
+        assert ost == oracle.OK and b.pivots(i) == ot == p, i
 
 
-def test_batch_group_traces_match_single(ctx_for, mgcd, wl):
+def test_batch_group_traces_match_single(ctx_for, mgcd, oracle, wl):
     # Every pair of a grouped batch follows its own pivot sequence: the
     # iteration and digit counts equal those of the pair run alone.
     ctx = ctx_for(32)
     pairs = wl.headline(8, seed=3, nbits=9000, gbits=2000)
     b = mgcd.Batch.from_pairs(ctx, pairs, workgroup=256, slots=1, group=16)
     b.run()
+    mods = ctx.moduli()
     for i, (u, v) in enumerate(pairs):
         st, g, d, p = ctx.gcd_trace(max(u, v), min(u, v))
         nq, R, k = b.pair_stats(i)
         assert (R, k) == (len(p), len(d)) and b.result(i) == g
+        ost, od, ot = oracle.gcd_kernel(max(u, v), min(u, v), mods[:nq], 32)
+        assert b.pivots(i) == ot == p, i
