@@ -249,12 +249,13 @@ __device__ __forceinline__ double class_min(double x) {
 }
 
 // ---------------------------------------------------------------------------
-// Workgroup state.
+// Workgroup state: the small per-round fields of one group (Small), and the
+// staging area that the groups a workgroup runs share (Big).
 template <int T, int K, int B>
-struct Shared {
+struct Small {
   static constexpr int NW = T / 64;
-  // The small per-round fields come first: their LDS addresses stay within
-  // the 16-bit immediate offset of ds_read/ds_write.
+  // The small per-round fields come first in LDS: their addresses stay
+  // within the 16-bit immediate offset of ds_read/ds_write.
   double red[B][NW];          // per-wave minima
   double bb[B], pp[B];        // current pivot of each pair: value, modulus
   uint32_t pidx[B];           // its modulus index (NONE_IDX: none)
@@ -266,11 +267,13 @@ struct Shared {
   uint32_t retire[T * K];     // bit b: this part's modulus is pair b's pivot
   uint32_t retw[K][NW];       // per slot row and wave: OR of its retire words
   uint32_t item;
-  // Staging for (a) and (c), and during (b) the group's moduli (when N_Q
-  // fits), so that a pivot's modulus is an LDS read instead of a global one
-  // on the path between two rounds.  Group tilings stage 16 doubles per
-  // thread (fewer barriers and global round trips per staged chunk; at most
-  // two such workgroups share a CU's 160 KB).
+};
+// Staging for (a) and (c), and during (b) the group's moduli (when N_Q
+// fits), so that a pivot's modulus is an LDS read instead of a global one
+// on the path between two rounds.  Group tilings stage 16 doubles per thread
+// (fewer barriers and global round trips per staged chunk).
+template <int T, int K, int B>
+struct Big {
   static constexpr int TL = B >= 8 ? 16 * T : TILE;
   static constexpr uint32_t MODS = (TL * 8 + B * T * K * 2) / 4;
   union {
@@ -280,6 +283,11 @@ struct Shared {
     } st;
     uint32_t mods[MODS];         // (b): modulus of every index < N_Q
   } u;
+};
+template <int T, int K, int B>
+struct Shared {
+  Small<T, K, B> s;
+  Big<T, K, B> b;
 };
 
 struct Ctx {
@@ -305,21 +313,36 @@ struct Ctx {
 // the phase bit.  Once a part has given up it posts ABORT in every later
 // round, whatever it would have posted, so a peer still waiting for it
 // gives up too instead of reading a value of another kind.
+//
+// grid_post starts a round (this part's slot), grid_collect finishes it.  A
+// part posts round r + 1 only after collecting round r, so no part is more
+// than one round ahead of another, and the two rows of the ring suffice.
+__device__ __forceinline__ uint64_t round_tag(const Ctx& C) {
+  return (uint64_t)(((C.rnd - 1) % 2047u + 1u) | (C.phase << 11));
+}
+
 template <int B>
-__device__ double grid_min(double x, Ctx& C) {
+__device__ __forceinline__ void grid_post(double x, Ctx& C) {
   const uint32_t lane = __lane_id();
   C.rnd += 1;
-  const uint64_t tag = (uint64_t)(((C.rnd - 1) % 2047u + 1u) | (C.phase << 11));
+  const uint64_t tag = round_tag(C);
+  unsigned long long* row = C.bar + (size_t)(C.rnd & 1) * C.G * B;
+  if (lane < (uint32_t)B)
+    __hip_atomic_store(row + (size_t)C.g * B + lane,
+                       (unsigned long long)((tag << 52) | (C.abort ? ABORT : key_bits(x))), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int B>
+__device__ double grid_collect(Ctx& C) {
+  const uint32_t lane = __lane_id();
+  const uint64_t tag = round_tag(C);
   // A slot is ready when its round matches; a matching round of the other
   // phase means a peer has left the sequence, and counts as ABORT.
   auto ready = [&](uint64_t v) { return ((v >> 52) & 0x7FFu) == (tag & 0x7FFu); };
   auto aborted = [&](uint64_t v) { return (v >> 52) != tag || (v & PAYLOAD) == ABORT; };
   unsigned long long* row = C.bar + (size_t)(C.rnd & 1) * C.G * B;
   unsigned long long* relay = C.bar + (size_t)2 * C.G * B + (size_t)(C.rnd & 1) * B;
-  if (lane < (uint32_t)B)
-    __hip_atomic_store(row + (size_t)C.g * B + lane,
-                       (unsigned long long)((tag << 52) | (C.abort ? ABORT : key_bits(x))), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
   if (C.relay && C.g != 0) {
     uint64_t v = __hip_atomic_load(relay + lane % B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t spins = 0;
@@ -420,6 +443,12 @@ __device__ double grid_min(double x, Ctx& C) {
   return m;
 }
 
+template <int B>
+__device__ __forceinline__ double grid_min(double x, Ctx& C) {
+  grid_post<B>(x, C);
+  return grid_collect<B>(C);
+}
+
 // (a) Residues of NOP operands at once (x[o], n[o] limbs) for every slot
 // row in use: NOP * K independent Horner chains per lane keep the FP64
 // pipeline busy.  The operands are aligned at their least significant limb
@@ -475,47 +504,49 @@ struct PairState {
   uint64_t trace_off;  // per-iteration path loads nothing from its descriptor
 };
 
-// Records pivot (idx, val) as iteration niter of the pair: trace entry
-// (leader workgroup), and the next round's pivot in LDS.
-template <int T, int K, int B>
-__device__ __forceinline__ void publish_pivot(const KernelArgs& args, bool lead, bool lds_mods, uint32_t b,
-                                              PairState& ps, uint32_t idx, int32_t val, Shared<T, K, B>& S) {
-  const uint32_t mu = lds_mods ? S.u.mods[idx] : args.mod_u32[idx];
-  if (lead && ps.niter < ps.trace_cap) {
-    mgcd_pair_t e;
-    e.modulus = mu;
-    e.value = val;
-    args.trace[ps.trace_off + ps.niter] = e;
-  }
-  ps.niter++;
-  ps.pval = val;
-  S.bb[b] = (double)val;
-  S.pp[b] = (double)mu;
-  S.pidx[b] = idx;
-}
-
-// Everything one workgroup does for part C.g of a group of up to B pairs.
+// Everything one workgroup does for part C.g of a group of up to B pairs,
+// phase by phase: (a) residues, (b) elimination rounds, (c) replay and
+// recovery, run in order by process_group.  S: the group's small LDS fields;
+// U: the staging area (and, during (b), the moduli table).
 template <int T, int K, int B, class A>
-__device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Shared<T, K, B>& S, Ctx& C) {
-  constexpr int NW = T / 64;
-  constexpr int SH = 6 - ilog2(B);  // lane l of a reduce-scatter holds pair l >> SH
-  const uint32_t tid = threadIdx.x;
-  const uint32_t lane = tid & 63, wave = tid >> 6;
-  const uint32_t nq = Gd.nq;
-  const uint32_t npairs = Gd.count;
+struct GroupRun {
+  static constexpr int NW = T / 64;
+  static constexpr int SH = 6 - ilog2(B);  // lane l of a reduce-scatter holds pair l >> SH
+  using SmallT = Small<T, K, B>;
+  using BigT = Big<T, K, B>;
+
+  const KernelArgs& args;
+  const GroupDesc Gd;
+  SmallT& S;
+  BigT& U;
+  Ctx& C;
+  PhaseClock& clk;
+  const uint32_t tid, lane, wave, nq, npairs;
   // Part g owns moduli [g*nq/G, (g+1)*nq/G): sizes differ by at most one,
   // none is empty (G <= nq), none exceeds T*K (checked by the host).
-  const uint32_t base = (uint32_t)(((uint64_t)C.g * nq) / C.G);
-  const uint32_t cnt = (uint32_t)(((uint64_t)(C.g + 1) * nq) / C.G) - base;
-  const int rows = (int)((cnt + T - 1) / T);
-  const bool lead = C.g == 0;
-  const int L = args.L;
-  PhaseClock clk;
-  clk.start();
+  const uint32_t base, cnt;
+  const int rows;
+  const bool lead;
+  const int L;
+  bool lds_mods = false;
+  ModD M[K];
+  double ck[K];
+  double t[B][K];
+  double wkey = KNONE;
+  uint32_t infw = 0;  // bit k: some lane of this wave holds TINF in slot row k (uniform)
+  PairState ps;
 
-  // ---- (a) residues of u and v, stored canonical for (c) -----------------
+  __device__ __forceinline__ GroupRun(const KernelArgs& a, const GroupDesc& gd, SmallT& s, BigT& u, Ctx& c, PhaseClock& ck_)
+      : args(a), Gd(gd), S(s), U(u), C(c), clk(ck_), tid(threadIdx.x), lane(threadIdx.x & 63),
+        wave(threadIdx.x >> 6), nq(gd.nq), npairs(gd.count),
+        base((uint32_t)(((uint64_t)c.g * gd.nq) / c.G)),
+        cnt((uint32_t)(((uint64_t)(c.g + 1) * gd.nq) / c.G) - (uint32_t)(((uint64_t)c.g * gd.nq) / c.G)),
+        rows((int)(((uint32_t)(((uint64_t)(c.g + 1) * gd.nq) / c.G) - (uint32_t)(((uint64_t)c.g * gd.nq) / c.G) + T - 1) / T)),
+        lead(c.g == 0), L(a.L) {}
+
+  // ---- (a) residues of u and v, stored canonical for (c) -------------------
   // The residues of part p (PP pairs, 2 PP operands, at a time).
-  auto residues_of_part = [&](uint32_t p) {
+  __device__ __forceinline__ void residues_of_part(uint32_t p) {
     const uint32_t pb = (uint32_t)(((uint64_t)p * nq) / C.G);
     const uint32_t pc = (uint32_t)(((uint64_t)(p + 1) * nq) / C.G) - pb;
     ModD PM[K];
@@ -542,7 +573,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         ro[2 * q + 1] = P.res_off + nq;
       }
       double r[2 * PP][K];
-      residues<T, K, 2 * PP, Shared<T, K, B>::TL>(x, nx, r, PM, (int)((pc + T - 1) / T), S.u.st.tile);
+      residues<T, K, 2 * PP, BigT::TL>(x, nx, r, PM, (int)((pc + T - 1) / T), U.u.st.tile);
 #pragma unroll
       for (int o = 0; o < 2 * PP; ++o) {
         if (b0 + o / 2 >= npairs) break;
@@ -551,10 +582,14 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
           if ((uint32_t)k * T + tid < pc) args.res[ro[o] + pb + k * T + tid] = (uint32_t)canon(r[o][k], PM[k]);
       }
     }
-  };
-  if (C.G == 1) {
-    residues_of_part(0);
-  } else {
+  }
+
+  __device__ __forceinline__ void residues_phase() {
+    if (npairs == 0) return;
+    if (C.G == 1) {
+      residues_of_part(0);
+      return;
+    }
     // Every part's residue job has a state word after the group's exchange
     // slots: 0 free, 1 taken, 2 done.  A part takes its own job, or waits
     // for the part that took it (its results are the same either way, so
@@ -625,97 +660,116 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
     }
   }
 
-  ModD M[K];
-  double ck[K];
+  // ---- (b) elimination -------------------------------------------------------
+  __device__ __forceinline__ void elim_setup() {
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const uint32_t j = (uint32_t)k * T + tid;
-    if (j < cnt) {
-      M[k].m = args.mod_d[base + j];
-      M[k].minv = args.mod_inv[base + j];
-    } else {
-      // No modulus here: the lane's values are all sentinels, which enter
-      // the elimination's products as they are, so it takes a table
-      // modulus (a small one would make such a product 0 often).
-      M[k].m = args.mod_d[0];
-      M[k].minv = args.mod_inv[0];
-    }
-    ck[k] = 0.5 + 2.0 * (double)(IDX_MAX - (base + j));
-  }
-
-  // ---- (b) elimination ----------------------------------------------------
-  clk.lap(PH_RESIDUES);
-  // The moduli table of the group in LDS, over (a)'s staging tile.
-  const bool lds_mods = nq <= Shared<T, K, B>::MODS;
-  __syncthreads();
-  if (lds_mods)
-    for (uint32_t i = tid; i < nq; i += T) S.u.mods[i] = args.mod_u32[i];
-  // Initial t = u/v (getGcdKernel.cu:369), one batched inverse per slot.
-  // Keys are reduced over the wave slot by slot (reduce-scatter), so only
-  // one slot's B keys are ever held: wkey in lane l is the wave's minimum
-  // for pair l >> SH.
-  double t[B][K];
-  double wkey = KNONE;
-  // Bit k: some lane of this wave holds TINF in slot row k (uniform).
-  uint32_t infw = 0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const uint32_t j = (uint32_t)k * T + tid;
-    const bool here = j < cnt;
-    double P[B], key[B];
-    bool anyinf = false;
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-      double v = 0.0;
-      if (here && (uint32_t)b < npairs) v = (double)args.res[args.pairs[Gd.first + b].res_off + nq + base + j];
-      const double y = v != 0.0 ? v : 1.0;
-      P[b] = b == 0 ? y : A::mulmod(P[b - 1], y, M[k]);
-    }
-    double r = A::modinv(P[B - 1], M[k]);
-#pragma unroll
-    for (int b = B - 1; b >= 0; --b) {
-      double u = 0.0, v = 0.0;
-      if (here && (uint32_t)b < npairs) {
-        const uint64_t ro = args.pairs[Gd.first + b].res_off + base + j;
-        u = (double)args.res[ro];
-        v = (double)args.res[ro + nq];
+    for (int k = 0; k < K; ++k) {
+      const uint32_t j = (uint32_t)k * T + tid;
+      if (j < cnt) {
+        M[k].m = args.mod_d[base + j];
+        M[k].minv = args.mod_inv[base + j];
+      } else {
+        // No modulus here: the lane's values are all sentinels, which enter
+        // the elimination's products as they are, so it takes a table
+        // modulus (a small one would make such a product 0 often).
+        M[k].m = args.mod_d[0];
+        M[k].minv = args.mod_inv[0];
       }
-      const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
-      if (b > 0) r = A::mulmod(r, v != 0.0 ? v : 1.0, M[k]);
-      double tv = v != 0.0 ? tsigned(A::mulmod(u, inv, M[k]), M[k]) : (u != 0.0 ? TINF : TZERO);
-      if (!here) tv = TRET;
-      anyinf |= tv == TINF;
-      t[b][k] = tv;
-      key[b] = elim_key(tv, ck[k]);
+      ck[k] = 0.5 + 2.0 * (double)(IDX_MAX - (base + j));
     }
-    if (__ballot(anyinf)) infw |= 1u << k;
-    wkey = __builtin_fmin(wkey, wave_min_scatter<B>(key));
+    lds_mods = nq <= BigT::MODS;
   }
 
-  // Per-pair state lives in lane b of wave 0.
-  PairState ps;
-  ps.live = 0;
-  ps.status = KST_NOTRUN;
-  ps.remaining = ps.ubits = ps.vbits = ps.pval = 0;
-  ps.niter = ps.ndig = 0;
-  ps.trace_cap = 0;
-  ps.trace_off = 0;
-  if (wave == 0 && lane < npairs) {
-    const PairDesc& Pl = args.pairs[Gd.first + lane];
-    ps.live = 1;
-    ps.status = KST_OK;
-    ps.remaining = (int32_t)nq;                   // totalModuliRemaining (:350)
-    ps.ubits = (int32_t)((Pl.nu + 1) * 32);       // :351
-    ps.vbits = (int32_t)((Pl.nv + 1) * 32);       // :352
-    ps.trace_cap = Pl.trace_cap;
-    ps.trace_off = Pl.trace_off;
+  // The moduli table of the group in LDS, over (a)'s staging tile.
+  __device__ __forceinline__ void load_mods() {
+    __syncthreads();
+    if (lds_mods)
+      for (uint32_t i = tid; i < nq; i += T) U.u.mods[i] = args.mod_u32[i];
   }
 
-  // Reduces the wave minima over the group; wave 0 lane b < B gets pair b's.
-  auto reduce_keys = [&](double w) -> double {
+  // Initial t = u/v (getGcdKernel.cu:369), one batched inverse per slot.
+  // Keys are reduced over the wave slot by slot (reduce-scatter), so only one
+  // slot's B keys are ever held: wkey in lane l is the wave's minimum for
+  // pair l >> SH.  Also the per-pair state (lane b of wave 0) and the retire
+  // lists.
+  __device__ __forceinline__ void elim_init() {
+    wkey = KNONE;
+    infw = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t j = (uint32_t)k * T + tid;
+      const bool here = j < cnt;
+      double P[B], key[B];
+      bool anyinf = false;
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        double v = 0.0;
+        if (here && (uint32_t)b < npairs) v = (double)args.res[args.pairs[Gd.first + b].res_off + nq + base + j];
+        const double y = v != 0.0 ? v : 1.0;
+        P[b] = b == 0 ? y : A::mulmod(P[b - 1], y, M[k]);
+      }
+      double r = A::modinv(P[B - 1], M[k]);
+#pragma unroll
+      for (int b = B - 1; b >= 0; --b) {
+        double u = 0.0, v = 0.0;
+        if (here && (uint32_t)b < npairs) {
+          const uint64_t ro = args.pairs[Gd.first + b].res_off + base + j;
+          u = (double)args.res[ro];
+          v = (double)args.res[ro + nq];
+        }
+        const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
+        if (b > 0) r = A::mulmod(r, v != 0.0 ? v : 1.0, M[k]);
+        double tv = v != 0.0 ? tsigned(A::mulmod(u, inv, M[k]), M[k]) : (u != 0.0 ? TINF : TZERO);
+        if (!here) tv = TRET;
+        anyinf |= tv == TINF;
+        t[b][k] = tv;
+        key[b] = elim_key(tv, ck[k]);
+      }
+      if (__ballot(anyinf)) infw |= 1u << k;
+      wkey = __builtin_fmin(wkey, wave_min_scatter<B>(key));
+    }
+
+    ps.live = 0;
+    ps.status = KST_NOTRUN;
+    ps.remaining = ps.ubits = ps.vbits = ps.pval = 0;
+    ps.niter = ps.ndig = 0;
+    ps.trace_cap = 0;
+    ps.trace_off = 0;
+    if (wave == 0 && lane < npairs) {
+      const PairDesc& Pl = args.pairs[Gd.first + lane];
+      ps.live = 1;
+      ps.status = KST_OK;
+      ps.remaining = (int32_t)nq;                   // totalModuliRemaining (:350)
+      ps.ubits = (int32_t)((Pl.nu + 1) * 32);       // :351
+      ps.vbits = (int32_t)((Pl.nv + 1) * 32);       // :352
+      ps.trace_cap = Pl.trace_cap;
+      ps.trace_off = Pl.trace_off;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) S.retire[k * T + tid] = 0;
+    if (tid < (uint32_t)(K * NW)) (&S.retw[0][0])[tid] = 0;
+  }
+
+  // This wave's minima into LDS (before the barrier that ends a round's compute).
+  __device__ __forceinline__ void stage_keys() {
     clk.lap(PH_COMPUTE);
     clk.round();
-    if ((lane & ((1u << SH) - 1)) == 0) S.red[lane >> SH][wave] = w;
+    if ((lane & ((1u << SH) - 1)) == 0) S.red[lane >> SH][wave] = wkey;
+  }
+  // After that barrier, wave 0: the workgroup's minimum of pair lane % B... in lane b < B.
+  __device__ __forceinline__ double workgroup_keys() {
+    double x = KNONE;
+    if (lane < (uint32_t)B) {
+      x = S.red[lane][0];
+#pragma unroll
+      for (int w2 = 1; w2 < NW; ++w2) x = __builtin_fmin(x, S.red[lane][w2]);
+      x = __builtin_fmin(x, KINF);
+    }
+    return x;
+  }
+  // Reduces the wave minima over the group; wave 0 lane b < B gets pair b's.
+  __device__ __forceinline__ double reduce_keys() {
+    stage_keys();
     __syncthreads();
     clk.lap(PH_BARRIER);
     double x = KNONE;
@@ -725,65 +779,89 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       // the waves of any other workgroup on the CU.  (One-pair tilings,
       // whose exchange dominates the round, measured slower with it.)
       if constexpr (B > 1) set_prio(3);
-      if (lane < (uint32_t)B) {
-        x = S.red[lane][0];
-#pragma unroll
-        for (int w2 = 1; w2 < NW; ++w2) x = __builtin_fmin(x, S.red[lane][w2]);
-        x = __builtin_fmin(x, KINF);
-      }
+      x = workgroup_keys();
       if (C.G > 1) x = grid_min<B>(x, C);
     }
     clk.lap(PH_EXCHANGE);
     return x;
-  };
+  }
 
-#pragma unroll
-  for (int k = 0; k < K; ++k) S.retire[k * T + tid] = 0;
-  if (tid < (uint32_t)(K * NW)) (&S.retw[0][0])[tid] = 0;
   // Marks pair b's pivot modulus for retirement if it lies in this part.
-  auto mark_retire = [&](uint32_t b, uint32_t idx) {
+  __device__ __forceinline__ void mark_retire(uint32_t b, uint32_t idx) {
     const uint32_t off = idx - base;
     if (off < cnt) {
       atomicOr(&S.retire[off], 1u << b);
       atomicOr(&S.retw[off / T][(off % T) >> 6], 1u << b);
     }
-  };
+  }
 
-  // First pivot: the loop body runs once even for an infinite one (do-while, :374-402).
-  {
-    const double kr = reduce_keys(wkey);
-    if (wave == 0) {
-      if (lane < npairs) {
+  // Records pivot (idx, val) as iteration niter of the pair: trace entry
+  // (leader workgroup), and the next round's pivot in LDS.
+  __device__ __forceinline__ void publish_pivot(uint32_t b, uint32_t idx, int32_t val) {
+    const uint32_t mu = lds_mods ? U.u.mods[idx] : args.mod_u32[idx];
+    if (lead && ps.niter < ps.trace_cap) {
+      mgcd_pair_t e;
+      e.modulus = mu;
+      e.value = val;
+      args.trace[ps.trace_off + ps.niter] = e;
+    }
+    ps.niter++;
+    ps.pval = val;
+    S.bb[b] = (double)val;
+    S.pp[b] = (double)mu;
+    S.pidx[b] = idx;
+  }
+
+  // Wave 0, after a round's minima kr (lane b: pair b's) are known: the
+  // first round's pivot (the loop body runs once even for an infinite one,
+  // do-while :374-402) or a later one with the size bookkeeping.
+  __device__ __forceinline__ void book(double kr, bool first) {
+    if (lane < npairs && ps.live) {
+      bool take = true;
+      if (!first) {
+        ps.remaining -= 1;                                          // :390
+        const int32_t tbits = ps.ubits - (L - 1) + ffs_abs(ps.pval);  // :391
+        ps.ubits = ps.vbits;
+        ps.vbits = tbits;
+        if (ps.remaining * (L - 2) <= ps.ubits) {                   // :393-399
+          ps.status = KST_REDUX;
+          ps.live = 0;
+          take = false;
+        }
+      }
+      if (take) {
         uint32_t idx;
         int32_t val;
-        // An infinite first pivot is recorded too; a malformed key (only
-        // ever after a peer gave up) ends the pair like a timeout.
-        if (kr == KNONE || decode_pivot(kr, nq, idx, val) == KEY_BAD) {
+        // A malformed key (only ever after a peer gave up) ends the pair
+        // like a timeout.
+        const int kind = kr == KNONE ? KEY_BAD : decode_pivot(kr, nq, idx, val);
+        if (kind == KEY_BAD) {
           ps.status = KST_TIMEOUT;
           ps.live = 0;
+        } else if (kind == KEY_INF && !first) {
+          ps.live = 0;  // infinity: done (:402)
         } else {
-          publish_pivot(args, lead, lds_mods, lane, ps, idx, val, S);
+          publish_pivot(lane, idx, val);
           mark_retire(lane, idx);
         }
       }
-      if (lane < (uint32_t)B && !ps.live) {
-        // A pair that is done (or absent) keeps being updated with t' =
-        // 0 / (t - 1): its finite t settle at 0, whose y = -1 is never 0
-        // (mod m), so it sends no wave through the full forward pass.
-        S.bb[lane] = 1.0;
-        S.pp[lane] = 0.0;
-        S.pidx[lane] = NONE_IDX;
-      }
-      const bool any = __ballot(ps.live != 0) != 0;
-      if (lane == 0) S.flag = any;
-      if constexpr (B > 1) set_prio(0);
     }
-    clk.lap(PH_BOOK);
-    __syncthreads();
-    clk.lap(PH_BARRIER2);
+    if (lane < (uint32_t)B && !ps.live) {
+      // A pair that is done (or absent) keeps being updated with t' =
+      // 0 / (t - 1): its finite t settle at 0, whose y = -1 is never 0
+      // (mod m), so it sends no wave through the full forward pass.
+      S.bb[lane] = 1.0;
+      S.pp[lane] = 0.0;
+      S.pidx[lane] = NONE_IDX;
+    }
+    const bool any = __ballot(ps.live != 0) != 0;
+    if (lane == 0) S.flag = any;
   }
 
-  while (S.flag) {
+  // One round's arithmetic on every slot row: retire the pivots' moduli,
+  // then t' = p' / (t - b) for every pair (:376-385); leaves the wave's
+  // minima in wkey.
+  __device__ __forceinline__ void compute_round() {
     // Retire each pair's pivot modulus (:376-377).  Few waves hold one; a
     // wave that does visits only the pairs named in its summary word
     // (uniform), with one select per pair.
@@ -808,9 +886,8 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         }
       }
     }
-    // t' = p' / (t - b) for every pair (:379-385): prefix products of the
-    // values to invert, one inverse, then back down (Montgomery's batch
-    // inversion).
+    // Prefix products of the values to invert, one inverse, then back down
+    // (Montgomery's batch inversion).
     wkey = KNONE;
     LdsPtr<double> bbp = lds_ptr(S.bb), ppp = lds_ptr(S.pp);
 #pragma unroll
@@ -838,7 +915,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
             const bool pole = fin & ((d == 0.0) | (__builtin_fabs(d) == M[k].m));
             if (pole) t[b][k] = TPOLE;
             pole_any |= pole;
-            y = fin & !pole ? d : 1.0;
+            y = (fin & !pole) ? d : 1.0;
           } else if constexpr (!A::kWideOperands) {
             y = __builtin_fabs(tv) < TFIN ? d : 1.0;
           }
@@ -911,76 +988,64 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       wkey = __builtin_fmin(wkey, wave_min_scatter<B>(key));
     }
     __builtin_amdgcn_s_setprio(0);
-    const double kr = reduce_keys(wkey);
-    if (wave == 0) {
-      if (lane < npairs && ps.live) {
-        ps.remaining -= 1;                                          // :390
-        const int32_t tbits = ps.ubits - (L - 1) + ffs_abs(ps.pval);  // :391
-        ps.ubits = ps.vbits;
-        ps.vbits = tbits;
-        if (ps.remaining * (L - 2) <= ps.ubits) {                   // :393-399
-          ps.status = KST_REDUX;
-          ps.live = 0;
-        } else {
-          uint32_t idx;
-          int32_t val;
-          const int kind = kr == KNONE ? KEY_BAD : decode_pivot(kr, nq, idx, val);
-          if (kind == KEY_BAD) {
-            ps.status = KST_TIMEOUT;
-            ps.live = 0;
-          } else if (kind == KEY_INF) {
-            ps.live = 0;  // infinity: done (:402)
-          } else {
-            publish_pivot(args, lead, lds_mods, lane, ps, idx, val, S);
-            mark_retire(lane, idx);
-          }
-        }
+  }
+
+  // The whole elimination of one group, round after round.
+  __device__ __forceinline__ void elimination() {
+    elim_init();
+    {
+      const double kr = reduce_keys();
+      if (wave == 0) {
+        book(kr, true);
+        if constexpr (B > 1) set_prio(0);
       }
-      if (lane < (uint32_t)B && !ps.live) {
-        // A pair that is done (or absent) keeps being updated with t' =
-        // 0 / (t - 1): its finite t settle at 0, whose y = -1 is never 0
-        // (mod m), so it sends no wave through the full forward pass.
-        S.bb[lane] = 1.0;
-        S.pp[lane] = 0.0;
-        S.pidx[lane] = NONE_IDX;
-      }
-      const bool any = __ballot(ps.live != 0) != 0;
-      if (lane == 0) S.flag = any;
-      if constexpr (B > 1) set_prio(0);
+      clk.lap(PH_BOOK);
+      __syncthreads();
+      clk.lap(PH_BARRIER2);
     }
-    clk.lap(PH_BOOK);
+    while (S.flag) {
+      compute_round();
+      const double kr = reduce_keys();
+      if (wave == 0) {
+        book(kr, false);
+        if constexpr (B > 1) set_prio(0);
+      }
+      clk.lap(PH_BOOK);
+      __syncthreads();
+      clk.lap(PH_BARRIER2);
+    }
+  }
+
+  // Surviving moduli, status and iteration counts into LDS; the pivots the
+  // leader workgroup recorded become visible to all parts.
+  __device__ __forceinline__ void finish_elimination() {
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const uint64_t w = __ballot(t[b][k] != TRET);
+        if (lane == 0) S.actw[b][k][wave] = w;
+      }
+    if (wave == 0 && lane < (uint32_t)B) {
+      S.status[lane] = lane < npairs ? ps.status : KST_NOTRUN;
+      S.ntrace[lane] = ps.niter;
+    }
+    C.phase = 1;
+    if (C.G > 1 && wave == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      (void)grid_min<B>(0.0, C);
+    }
     __syncthreads();
-    clk.lap(PH_BARRIER2);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    clk.lap(PH_TAIL);
   }
 
-  // Surviving moduli of each pair, as ballot words.
-#pragma unroll
-  for (int b = 0; b < B; ++b)
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint64_t w = __ballot(t[b][k] != TRET);
-      if (lane == 0) S.actw[b][k][wave] = w;
-    }
-  if (wave == 0 && lane < (uint32_t)B) {
-    S.status[lane] = lane < npairs ? ps.status : KST_NOTRUN;
-    S.ntrace[lane] = ps.niter;
-  }
-  // The pivots recorded by the leader workgroup become visible to all parts.
-  C.phase = 1;
-  if (C.G > 1 && wave == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    (void)grid_min<B>(0.0, C);
-  }
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-
-  clk.lap(PH_TAIL);
-  // ---- (c1) replay of the pivots on the survivors -------------------------
+  // ---- (c1) replay of the pivots on the survivors ---------------------------
   // The survivors of every pair of the group form one compacted list
   // (pair-major; entry = pair << 11 | slot offset); each lane replays up to
   // four entries at a time, which may belong to different pairs: the pivots
   // of all pairs are staged through LDS together, RC steps at a time.
-  {
+  __device__ __forceinline__ void replay() {
     uint32_t nsurv = 0, maxR = 0;
     for (uint32_t b = 0; b < npairs; ++b) {
       if (S.status[b] != KST_OK) continue;
@@ -991,7 +1056,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         for (int w = 0; w < NW; ++w) {
           const uint64_t word = S.actw[b][k][w];
           if ((uint32_t)w == wave && ((word >> lane) & 1))
-            S.u.st.surv[nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(word >> 32),
+            U.u.st.surv[nsurv + __builtin_amdgcn_mbcnt_hi((uint32_t)(word >> 32),
                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)word, 0))] =
                 (uint16_t)((b << 11) | (uint32_t)(k * T + tid));
           nsurv += (uint32_t)__builtin_popcountll(word);
@@ -999,7 +1064,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
     }
     __syncthreads();
     constexpr int E = B >= 4 ? 4 : K;  // entries per lane per round
-    constexpr int RC = Shared<T, K, B>::TL / (2 * B);  // pivots per pair per staging
+    constexpr int RC = BigT::TL / (2 * B);  // pivots per pair per staging
     for (uint32_t e0 = 0; e0 < nsurv; e0 += E * T) {
       ModD RM[E];
       double ra[E], rc[E], rs[E];
@@ -1016,7 +1081,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
         eb[e] = 0;
         eR[e] = 0;
         if (j < nsurv) {
-          const uint32_t ent = S.u.st.surv[j];
+          const uint32_t ent = U.u.st.surv[j];
           const uint32_t idx = base + (ent & 0x7FFu);
           eb[e] = ent >> 11;
           eR[e] = S.ntrace[eb[e]];
@@ -1039,12 +1104,12 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
           const uint32_t b = j / len, jj = j - b * len;
           double bv = 0.0, pv = 1.0;
           if (S.status[b] == KST_OK && c0 + jj < S.ntrace[b]) {
-            const mgcd_pair_t t = args.trace[args.pairs[Gd.first + b].trace_off + c0 + jj];
-            bv = (double)t.value;
-            pv = (double)t.modulus;
+            const mgcd_pair_t tr = args.trace[args.pairs[Gd.first + b].trace_off + c0 + jj];
+            bv = (double)tr.value;
+            pv = (double)tr.modulus;
           }
-          S.u.st.tile[2 * (b * RC + jj)] = bv;
-          S.u.st.tile[2 * (b * RC + jj) + 1] = pv;
+          U.u.st.tile[2 * (b * RC + jj)] = bv;
+          U.u.st.tile[2 * (b * RC + jj) + 1] = pv;
         }
         __syncthreads();
         if (ew > 0) {
@@ -1057,7 +1122,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
 #pragma unroll
             for (int e = 0; e < E; ++e) {
               if (e < ew && c0 + j < eR[e]) {
-                const double bv = S.u.st.tile[2 * (eb[e] * RC + j)], pv = S.u.st.tile[2 * (eb[e] * RC + j) + 1];
+                const double bv = U.u.st.tile[2 * (eb[e] * RC + j)], pv = U.u.st.tile[2 * (eb[e] * RC + j) + 1];
                 const double c2 = ra[e] - A::mulmod(bv, rc[e], RM[e]);  // |c2| <= m + 2^14
                 ra[e] = A::mulmod(rc[e], pv, RM[e]);
                 rs[e] = A::mulmod(rs[e], pv, RM[e]);
@@ -1077,198 +1142,216 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
       }
     }
     __syncthreads();
+    clk.lap(PH_REPLAY);
   }
 
-  clk.lap(PH_REPLAY);
-  // ---- (c2) recovery, RB pairs at a time in lockstep (:404-433) -----------
+  // ---- (c2) recovery, RB pairs at a time in lockstep (:404-433) -------------
   // u = a/s per surviving modulus; a lane inverts s only for a digit it owns.
   // Chunk c holds pairs c*RB + j; wave 0 lane c*RB + j keeps pair c*RB + j's
   // bookkeeping and gets its minima from the slots of class j.
-  constexpr int RB = B < 8 ? B : 8;
-  if (wave == 0 && lane < npairs) ps.live = ps.status == KST_OK;
-  for (uint32_t c0 = 0; c0 < npairs; c0 += RB) {
-    double ua[RB][K], us[RB][K];
-    uint32_t am[RB];  // surviving slots (bits)
-#pragma unroll
-    for (int j = 0; j < RB; ++j) {
-      const uint32_t b = c0 + j;
-      am[j] = 0;
-      const bool run = b < npairs && S.status[b] == KST_OK;
-      const uint64_t ro = run ? args.pairs[Gd.first + b].res_off : 0;
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        ua[j][k] = 0.0;
-        us[j][k] = 1.0;
-        if (run && ((S.actw[b][k][wave] >> lane) & 1)) {
-          am[j] |= 1u << k;
-          const uint32_t idx = base + (uint32_t)k * T + tid;
-          ua[j][k] = (double)args.res[ro + idx];
-          us[j][k] = (double)args.res[ro + nq + idx];
-        }
-      }
-    }
-    const bool mine = wave == 0 && lane >= c0 && lane < c0 + RB;  // this lane's pair is in the chunk
-
-    // Selects each pair's next digit: the lowest index holding a nonzero u;
-    // wave 0 lane c0 + j gets (index, value) of pair c0 + j as a recovery key.
-    auto select = [&]() -> double {
-      constexpr int SHR = 6 - ilog2(RB);
-      double cand[RB];
+  __device__ __forceinline__ void recovery() {
+    constexpr int RB = B < 8 ? B : 8;
+    if (wave == 0 && lane < npairs) ps.live = ps.status == KST_OK;
+    for (uint32_t c0 = 0; c0 < npairs; c0 += RB) {
+      double ua[RB][K], us[RB][K];
+      uint32_t am[RB];  // surviving slots (bits)
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
-        cand[j] = (double)NONE_IDX;
-#pragma unroll
-        for (int k = K - 1; k >= 0; --k)
-          if (((am[j] >> k) & 1) && ua[j][k] != 0.0) cand[j] = (double)(base + (uint32_t)k * T + tid);
-      }
-      const double w = wave_min_scatter<RB>(cand);
-      if ((lane & ((1u << SHR) - 1)) == 0) S.red[lane >> SHR][wave] = w;
-      __syncthreads();
-      // Owners compute toSigned(u) (:424); one batched inverse per slot.
-      uint32_t own[K];  // bit j: this slot holds pair c0 + j's candidate
-#pragma unroll
-      for (int k = 0; k < K; ++k) own[k] = 0;
-#pragma unroll
-      for (int j = 0; j < RB; ++j) {
-        double x = S.red[j][0];
-#pragma unroll
-        for (int w2 = 1; w2 < NW; ++w2) x = __builtin_fmin(x, S.red[j][w2]);
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-          if (((am[j] >> k) & 1) && (double)(base + (uint32_t)k * T + tid) == x) own[k] |= 1u << j;
-        // No nonzero residue in this part: (NONE_IDX, 0), which the reference
-        // resolves to its thread 0 pair with value 0 (:57-64).
-        if ((uint32_t)j == tid && x == (double)NONE_IDX) S.sel[j] = KNODIGIT;
-      }
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        if (__ballot(own[k] != 0) == 0) continue;
-        const double idx = (double)(base + (uint32_t)k * T + tid);
-        double P[RB];
-#pragma unroll
-        for (int j = 0; j < RB; ++j) {
-          const double y = ((own[k] >> j) & 1) ? us[j][k] : 1.0;
-          P[j] = j == 0 ? y : A::mulmod(P[j - 1], y, M[k]);
-        }
-        double r = A::modinv(P[RB - 1], M[k]);
-#pragma unroll
-        for (int j = RB - 1; j >= 0; --j) {
-          const bool owns = (own[k] >> j) & 1;
-          const double inv = j == 0 ? r : A::mulmod(r, P[j - 1], M[k]);
-          if (j > 0) r = A::mulmod(r, owns ? us[j][k] : 1.0, M[k]);
-          if (owns) {
-            const double v = tsigned(A::mulmod(ua[j][k], inv, M[k]), M[k]);
-            S.sel[j] = idx * 0x1p32 + (v + 0x1p31);
-          }
-        }
-      }
-      __syncthreads();
-      double x = KNONE;
-      if (wave == 0) {
-        x = S.sel[lane % RB];
-        if (C.G > 1) x = grid_min<RB>(x, C);
-      }
-      return x;
-    };
-
-    // Publishes digit (idx, val) of pair b (this lane's) as the next update.
-    auto set_digit = [&](uint32_t idx, int32_t val) {
-      S.bb[lane - c0] = (double)val;
-      S.pp[lane - c0] = args.mod_d[idx];
-      S.pidx[lane - c0] = idx;
-      ps.pval = val;
-    };
-    // Pairs not (or no longer) in recovery publish an update that matches
-    // no modulus.
-    auto finish_round = [&]() {
-      if (wave == 0) {
-        const uint32_t b = c0 + lane % RB;
-        const bool running = __shfl(ps.live, (int)(b & 63)) != 0 && b < npairs;
-        if (lane < (uint32_t)RB) {
-          if (!running) {
-            S.bb[lane] = 0.0;
-            S.pp[lane] = 1.0;
-            S.pidx[lane] = NONE_IDX;
-          }
-        }
-        const bool any = __ballot(mine && ps.live != 0) != 0;
-        if (lane == 0) S.flag = any;
-      }
-      __syncthreads();
-    };
-
-    {
-      const double x = select();
-      if (mine && lane < npairs && ps.live) {
-        uint32_t idx;
-        int32_t val;
-        if (x == KNONE || decode_digit(x, nq, idx, val) == KEY_BAD) {
-          ps.status = KST_TIMEOUT;
-          ps.live = 0;
-        } else {
-          set_digit(idx, val);
-        }
-      }
-      finish_round();
-    }
-    while (S.flag) {
-      // u' = (u - g) / p' on every other surviving modulus (:417-425).
-#pragma unroll
-      for (int j = 0; j < RB; ++j) {
-        const uint32_t pidx = __builtin_amdgcn_readfirstlane(S.pidx[j]);
-        const double gv = S.bb[j], pv = S.pp[j];
+        const uint32_t b = c0 + j;
+        am[j] = 0;
+        const bool run = b < npairs && S.status[b] == KST_OK;
+        const uint64_t ro = run ? args.pairs[Gd.first + b].res_off : 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          if ((am[j] >> k) & 1) {
-            if (base + (uint32_t)k * T + tid == pidx) {
-              am[j] &= ~(1u << k);
-            } else {
-              ua[j][k] = reduce(ua[j][k] - A::mulmod(gv, us[j][k], M[k]), M[k]);
-              us[j][k] = A::mulmod(us[j][k], pv, M[k]);
+          ua[j][k] = 0.0;
+          us[j][k] = 1.0;
+          if (run && ((S.actw[b][k][wave] >> lane) & 1)) {
+            am[j] |= 1u << k;
+            const uint32_t idx = base + (uint32_t)k * T + tid;
+            ua[j][k] = (double)args.res[ro + idx];
+            us[j][k] = (double)args.res[ro + nq + idx];
+          }
+        }
+      }
+      const bool mine = wave == 0 && lane >= c0 && lane < c0 + RB;  // this lane's pair is in the chunk
+
+      // Selects each pair's next digit: the lowest index holding a nonzero u;
+      // wave 0 lane c0 + j gets (index, value) of pair c0 + j as a recovery key.
+      auto select = [&]() -> double {
+        constexpr int SHR = 6 - ilog2(RB);
+        double cand[RB];
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+          cand[j] = (double)NONE_IDX;
+#pragma unroll
+          for (int k = K - 1; k >= 0; --k)
+            if (((am[j] >> k) & 1) && ua[j][k] != 0.0) cand[j] = (double)(base + (uint32_t)k * T + tid);
+        }
+        const double w = wave_min_scatter<RB>(cand);
+        if ((lane & ((1u << SHR) - 1)) == 0) S.red[lane >> SHR][wave] = w;
+        __syncthreads();
+        // Owners compute toSigned(u) (:424); one batched inverse per slot.
+        uint32_t own[K];  // bit j: this slot holds pair c0 + j's candidate
+#pragma unroll
+        for (int k = 0; k < K; ++k) own[k] = 0;
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+          double x = S.red[j][0];
+#pragma unroll
+          for (int w2 = 1; w2 < NW; ++w2) x = __builtin_fmin(x, S.red[j][w2]);
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (((am[j] >> k) & 1) && (double)(base + (uint32_t)k * T + tid) == x) own[k] |= 1u << j;
+          // No nonzero residue in this part: (NONE_IDX, 0), which the reference
+          // resolves to its thread 0 pair with value 0 (:57-64).
+          if ((uint32_t)j == tid && x == (double)NONE_IDX) S.sel[j] = KNODIGIT;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (__ballot(own[k] != 0) == 0) continue;
+          const double idx = (double)(base + (uint32_t)k * T + tid);
+          double P[RB];
+#pragma unroll
+          for (int j = 0; j < RB; ++j) {
+            const double y = ((own[k] >> j) & 1) ? us[j][k] : 1.0;
+            P[j] = j == 0 ? y : A::mulmod(P[j - 1], y, M[k]);
+          }
+          double r = A::modinv(P[RB - 1], M[k]);
+#pragma unroll
+          for (int j = RB - 1; j >= 0; --j) {
+            const bool owns = (own[k] >> j) & 1;
+            const double inv = j == 0 ? r : A::mulmod(r, P[j - 1], M[k]);
+            if (j > 0) r = A::mulmod(r, owns ? us[j][k] : 1.0, M[k]);
+            if (owns) {
+              const double v = tsigned(A::mulmod(ua[j][k], inv, M[k]), M[k]);
+              S.sel[j] = idx * 0x1p32 + (v + 0x1p31);
             }
           }
         }
-      }
-      const double x = select();
-      if (mine && lane < npairs && ps.live) {
-        if (lead) {
-          mgcd_pair_t dp;
-          dp.modulus = args.mod_u32[S.pidx[lane - c0]];
-          dp.value = ps.pval;
-          args.digits[args.pairs[Gd.first + lane].dig_off + ps.ndig] = dp;
+        __syncthreads();
+        double x = KNONE;
+        if (wave == 0) {
+          x = S.sel[lane % RB];
+          if (C.G > 1) x = grid_min<RB>(x, C);
         }
-        ps.ndig++;
-        ps.remaining -= 1;                                       // :428
-        if (ps.remaining <= 0) {                                 // :429-430, :440
-          ps.status = ps.pval != 0 ? KST_RECOVERY : KST_OK;
-          ps.live = 0;
-        } else {
+        return x;
+      };
+
+      // Publishes digit (idx, val) of pair b (this lane's) as the next update.
+      auto set_digit = [&](uint32_t idx, int32_t val) {
+        S.bb[lane - c0] = (double)val;
+        S.pp[lane - c0] = args.mod_d[idx];
+        S.pidx[lane - c0] = idx;
+        ps.pval = val;
+      };
+      // Pairs not (or no longer) in recovery publish an update that matches
+      // no modulus.
+      auto finish_round = [&]() {
+        if (wave == 0) {
+          const uint32_t b = c0 + lane % RB;
+          const bool running = __shfl(ps.live, (int)(b & 63)) != 0 && b < npairs;
+          if (lane < (uint32_t)RB) {
+            if (!running) {
+              S.bb[lane] = 0.0;
+              S.pp[lane] = 1.0;
+              S.pidx[lane] = NONE_IDX;
+            }
+          }
+          const bool any = __ballot(mine && ps.live != 0) != 0;
+          if (lane == 0) S.flag = any;
+        }
+        __syncthreads();
+      };
+
+      {
+        const double x = select();
+        if (mine && lane < npairs && ps.live) {
           uint32_t idx;
           int32_t val;
           if (x == KNONE || decode_digit(x, nq, idx, val) == KEY_BAD) {
             ps.status = KST_TIMEOUT;
             ps.live = 0;
-          } else if (val == 0) {
-            ps.live = 0;  // :433
           } else {
             set_digit(idx, val);
           }
         }
+        finish_round();
       }
-      finish_round();
+      while (S.flag) {
+        // u' = (u - g) / p' on every other surviving modulus (:417-425).
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+          const uint32_t pidx = __builtin_amdgcn_readfirstlane(S.pidx[j]);
+          const double gv = S.bb[j], pv = S.pp[j];
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if ((am[j] >> k) & 1) {
+              if (base + (uint32_t)k * T + tid == pidx) {
+                am[j] &= ~(1u << k);
+              } else {
+                ua[j][k] = reduce(ua[j][k] - A::mulmod(gv, us[j][k], M[k]), M[k]);
+                us[j][k] = A::mulmod(us[j][k], pv, M[k]);
+              }
+            }
+          }
+        }
+        const double x = select();
+        if (mine && lane < npairs && ps.live) {
+          if (lead) {
+            mgcd_pair_t dp;
+            dp.modulus = args.mod_u32[S.pidx[lane - c0]];
+            dp.value = ps.pval;
+            args.digits[args.pairs[Gd.first + lane].dig_off + ps.ndig] = dp;
+          }
+          ps.ndig++;
+          ps.remaining -= 1;                                       // :428
+          if (ps.remaining <= 0) {                                 // :429-430, :440
+            ps.status = ps.pval != 0 ? KST_RECOVERY : KST_OK;
+            ps.live = 0;
+          } else {
+            uint32_t idx;
+            int32_t val;
+            if (x == KNONE || decode_digit(x, nq, idx, val) == KEY_BAD) {
+              ps.status = KST_TIMEOUT;
+              ps.live = 0;
+            } else if (val == 0) {
+              ps.live = 0;  // :433
+            } else {
+              set_digit(idx, val);
+            }
+          }
+        }
+        finish_round();
+      }
     }
-  }
 
-  if (lead && wave == 0 && lane < npairs) {
-    PairOut o;
-    o.status = ps.status;
-    o.ndig = ps.ndig;
-    o.ntrace = ps.niter;
-    o.pad = 0;
-    args.out[Gd.first + lane] = o;
+    if (lead && wave == 0 && lane < npairs) {
+      PairOut o;
+      o.status = ps.status;
+      o.ndig = ps.ndig;
+      o.ntrace = ps.niter;
+      o.pad = 0;
+      args.out[Gd.first + lane] = o;
+    }
+    clk.lap(PH_RECOVERY);
   }
-  clk.lap(PH_RECOVERY);
-  clk.flush(args.clocks, tid == 0);
+};
+
+// Everything one workgroup does for part C.g of a group of up to B pairs.
+template <int T, int K, int B, class A>
+__device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Shared<T, K, B>& S, Ctx& C) {
+  PhaseClock clk;
+  clk.start();
+  GroupRun<T, K, B, A> g(args, Gd, S.s, S.b, C, clk);
+  g.residues_phase();
+  clk.lap(PH_RESIDUES);
+  g.elim_setup();
+  g.load_mods();
+  g.elimination();
+  g.finish_elimination();
+  g.replay();
+  g.recovery();
+  clk.flush(args.clocks, threadIdx.x == 0);
 }
 
 // Next work item for this workgroup (thread 0): queue 0, then the queue of
@@ -1285,29 +1368,33 @@ __device__ uint32_t next_item(const KernelArgs& args, uint32_t own, uint32_t* do
   return ~0u;
 }
 
+__device__ __forceinline__ void start_ctx(Ctx& C, const KernelArgs& args, const GroupDesc& Gd, uint32_t part) {
+  C.spin_limit = args.spin_limit;
+  C.G = Gd.G;
+  C.g = part;
+  C.rnd = 0;
+  C.phase = 0;
+  C.abort = false;
+  C.relay = Gd.G >= args.relay_min_g;
+  C.bar = args.bar + Gd.bar_off;
+}
+
 template <int T, int K, int B, class A>
 __global__ __launch_bounds__(T) void gcd_kernel(KernelArgs args) {
   __shared__ Shared<T, K, B> S;
   Ctx C;
-  C.spin_limit = args.spin_limit;
   const uint32_t xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);  // hwreg(HW_REG_XCC_ID, 0, 4)
   const uint32_t own = args.nqueues > 1 ? xcc % (args.nqueues - 1) : 0;
   uint32_t done = 0;
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) S.item = next_item(args, own, &done);
+    if (threadIdx.x == 0) S.s.item = next_item(args, own, &done);
     __syncthreads();
-    const uint32_t it = S.item;
+    const uint32_t it = S.s.item;
     if (it == ~0u) break;
     const WorkItem W = args.items[it];
     const GroupDesc Gd = args.groups[W.group];
-    C.G = Gd.G;
-    C.g = W.part;
-    C.rnd = 0;
-    C.phase = 0;
-    C.abort = false;
-    C.relay = Gd.G >= args.relay_min_g;
-    C.bar = args.bar + Gd.bar_off;
+    start_ctx(C, args, Gd, W.part);
     process_group<T, K, B, A>(args, Gd, S, C);
   }
 }
