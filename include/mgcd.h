@@ -105,10 +105,11 @@ enum {
    * on its first run and replays the graph afterwards.  0 (default):
    * cooperative launches. */
   MGCD_OPT_BATCH_GRAPH = 6,
-  /* Pairs split over at least this many workgroups (default 48, >= 2)
-   * exchange their per-iteration minima through part 0 (two hops, traffic
-   * linear in the parts) instead of every part reading every part's slot
-   * (one hop, traffic quadratic in the parts). */
+  /* Pairs split over at least this many workgroups (>= 2; default 48 for
+   * groups of pairs, 128 for one-pair launches) exchange their per-iteration
+   * minima through part 0 (two hops, traffic linear in the parts) instead of
+   * every part reading every part's slot (one hop, traffic quadratic in the
+   * parts). */
   MGCD_OPT_RELAY_PARTS = 7,
   /* 1 (default): the signed mixed-radix digits become positional limbs on
    * the device, by a product tree (batches; one-pair calls above 64 digits),

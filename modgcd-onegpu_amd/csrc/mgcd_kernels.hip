@@ -1502,15 +1502,16 @@ __global__ void certify_rcp_kernel(unsigned long long* fail, unsigned long long*
 #define MGCD_VARIANT(T, K, B, A, W) {T, K, B, W, reinterpret_cast<const void*>(&gcd_kernel<T, K, B, A>)}
 #ifdef MGCD_PHASE_CLOCKS
 // The instrumented build: the one-pair tilings and the default batch tiling.
-#define MGCD_VARIANTS(A, W) \
-  MGCD_VARIANT(256, 1, 1, A, W), MGCD_VARIANT(1024, 1, 1, A, W), MGCD_VARIANT(512, 2, 16, A, W)
+#define MGCD_VARIANTS(A, W)                                                                      \
+  MGCD_VARIANT(256, 1, 1, A, W), MGCD_VARIANT(1024, 1, 1, A, W), MGCD_VARIANT(512, 2, 16, A, W), \
+      MGCD_VARIANT(512, 1, 1, A, W)
 #else
 #define MGCD_VARIANTS(A, W)                                                                              \
   MGCD_VARIANT(256, 1, 1, A, W), MGCD_VARIANT(256, 2, 1, A, W), MGCD_VARIANT(256, 4, 1, A, W),           \
       MGCD_VARIANT(256, 8, 1, A, W), MGCD_VARIANT(1024, 1, 1, A, W), MGCD_VARIANT(1024, 2, 1, A, W),     \
       MGCD_VARIANT(256, 1, 8, A, W), MGCD_VARIANT(256, 2, 8, A, W), MGCD_VARIANT(256, 1, 16, A, W),      \
       MGCD_VARIANT(256, 2, 16, A, W), MGCD_VARIANT(256, 1, 32, A, W), MGCD_VARIANT(128, 1, 16, A, W),    \
-      MGCD_VARIANT(512, 2, 16, A, W), MGCD_VARIANT(512, 1, 16, A, W)
+      MGCD_VARIANT(512, 2, 16, A, W), MGCD_VARIANT(512, 1, 16, A, W), MGCD_VARIANT(512, 1, 1, A, W)
 #endif
 static const KernelVariant kVariants[] = {MGCD_VARIANTS(Arith32, 1), MGCD_VARIANTS(Arith22, 0)};
 #undef MGCD_VARIANTS

@@ -33,6 +33,12 @@ namespace {
 
 constexpr uint32_t kMaxModuli = (1u << 19) - 1;  // kernel key field width
 constexpr uint32_t kSpinLimit = 1u << 24;        // polls before a wait reports a timeout
+// Default relay thresholds (parts): groups of pairs relay from 48 parts (the
+// quadratic polling of ~160-part groups slowed every kernel on the device);
+// one pair's parts poll one slot each, and relaying paid only from ~128 parts
+// (measured: flat faster at 31-80 parts, the relay faster at 317).
+constexpr uint32_t kRelayGroups = 48;
+constexpr uint32_t kRelayOnePair = 128;
 
 const char* kMessages[] = {
     "OK",
@@ -384,7 +390,7 @@ struct mgcd_ctx {
   bool autotune = false;  // MGCD_OPT_AUTOTUNE
   bool batch_graph = false;  // MGCD_OPT_BATCH_GRAPH
   bool tune_loaded = false;
-  uint32_t relay_min_g = 48;  // MGCD_OPT_RELAY_PARTS
+  uint32_t relay_min_g = 0;   // MGCD_OPT_RELAY_PARTS (0: kRelayGroups / kRelayOnePair)
   bool device_convert = true;  // MGCD_OPT_DEVICE_CONVERT
   uint32_t spin_limit = kSpinLimit;  // MGCD_OPT_SPIN_LIMIT
   bool cooperative = true;     // MGCD_OPT_COOPERATIVE (default: $MGCD_COOPERATIVE, else 1)
@@ -541,6 +547,12 @@ int launch(mgcd_ctx* ctx, const LaunchSet& ls, const KernelArgs& ka) {
   return MGCD_OK;
 }
 
+// The relay threshold a launch of variant vi uses.
+uint32_t relay_parts(const mgcd_ctx* ctx, int vi) {
+  if (ctx->relay_min_g) return ctx->relay_min_g;
+  return kernel_variant(vi).B == 1 ? kRelayOnePair : kRelayGroups;
+}
+
 int choose_single_variant(mgcd_ctx* ctx, uint32_t nq) {
   const int w = ctx->wide;
   if (ctx->single_T > 0 && ctx->single_K > 0) {
@@ -550,6 +562,16 @@ int choose_single_variant(mgcd_ctx* ctx, uint32_t nq) {
       return vi;
   }
   if (nq <= 1024) return find_kernel_variant(1024, 1, 1, w);
+  // From 48 parts of 256 threads up to 128 parts of 512 threads the
+  // 512-thread tiling (two waves per SIMD, half the parts, no relay) is
+  // faster (measured 10-15 % at 128-256 Kibit, 7-12 % at 352 Kibit); with
+  // more parts (1 Mbit) the 256-thread one, relayed.
+  {
+    const int v512 = find_kernel_variant(512, 1, 1, w);
+    const uint32_t g256 = (nq + 255) / 256, g512 = (nq + 511) / 512;
+    if (v512 >= 0 && g256 >= kRelayGroups && g512 < kRelayOnePair && g512 <= (uint32_t)ctx->max_resident[v512])
+      return v512;
+  }
   for (int K : {1, 2, 4, 8}) {
     const int vi = find_kernel_variant(256, K, 1, w);
     if (vi < 0) continue;
@@ -833,7 +855,7 @@ int gcd_single(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, s
   ka.res = ctx->res.as<uint32_t>();
   ka.out = ctx->out.as<PairOut>();
   ka.spin_limit = ctx->spin_limit;
-  ka.relay_min_g = ctx->relay_min_g;
+  ka.relay_min_g = relay_parts(ctx, vi);
   ka.clocks = ctx->clocks.as<unsigned long long>();
   rc = launch(ctx, ls, ka);
   if (rc != MGCD_OK) return rc;
@@ -939,7 +961,7 @@ int gcd_single_noalloc(mgcd_ctx* ctx, unsigned char* base, const uint32_t* u, si
   ka.res = reinterpret_cast<uint32_t*>(base + kScratch.res);
   ka.out = reinterpret_cast<PairOut*>(base + kScratch.out);
   ka.spin_limit = ctx->spin_limit;
-  ka.relay_min_g = ctx->relay_min_g;
+  ka.relay_min_g = relay_parts(ctx, vi);
   int rc = launch(ctx, ls, ka);  // clears the queue counters and exchange slots with fill kernels
   if (rc != MGCD_OK) return rc;
   PairOut po;
@@ -1220,7 +1242,7 @@ KernelArgs batch_args(const mgcd_batch* b, const BatchLaunch& l) {
   ka.res = b->res.as<uint32_t>();
   ka.out = b->out.as<PairOut>();
   ka.spin_limit = ctx->spin_limit;
-  ka.relay_min_g = ctx->relay_min_g;
+  ka.relay_min_g = relay_parts(ctx, l.variant);
   ka.clocks = ctx->clocks.as<unsigned long long>();
   ka.nqueues = l.ls.nqueues;
   std::memcpy(ka.qoff, l.ls.qoff, sizeof ka.qoff);
@@ -1983,7 +2005,7 @@ int server_launch(mgcd_server* s, ServerSlot& sl) {
   ka.res = sl.res.as<uint32_t>();
   ka.out = sl.out.as<PairOut>();
   ka.spin_limit = ctx->spin_limit;
-  ka.relay_min_g = ctx->relay_min_g;
+  ka.relay_min_g = relay_parts(ctx, vi);
   ka.nqueues = sl.ls.nqueues;
   std::memcpy(ka.qoff, sl.ls.qoff, sizeof ka.qoff);
   HIP_TRY(launch_fill_u32(ka.queue, kMaxQueues * kQueueStride, 0u, st));

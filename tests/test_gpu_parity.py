@@ -180,7 +180,7 @@ def test_batch_alone_vs_mixed(ctx_for, mgcd, wl):
     assert alone.result(0) == together.result(10) == math.gcd(*probe)
 
 
-@pytest.mark.parametrize("T,K,B", [(256, 1, 1), (256, 4, 1), (1024, 2, 1), (256, 1, 8), (256, 2, 8),
+@pytest.mark.parametrize("T,K,B", [(256, 1, 1), (256, 4, 1), (512, 1, 1), (1024, 2, 1), (256, 1, 8), (256, 2, 8),
                                    (256, 1, 16), (256, 2, 16), (256, 1, 32), (128, 1, 16)])
 def test_batch_tilings_agree(ctx_for, mgcd, wl, T, K, B):
     # Groups of B pairs with equal N_Q share one modular inverse per modulus;
@@ -234,3 +234,26 @@ def test_batch_group_traces_match_single(ctx_for, mgcd, oracle, wl):
         assert (R, k) == (len(p), len(d)) and b.result(i) == g
         ost, od, ot = oracle.gcd_kernel(max(u, v), min(u, v), mods[:nq], 32)
         assert b.pivots(i) == ot == p, i
+
+
+def test_one_pair_512_thread_tiling(ctx_for, mgcd, oracle, wl):
+    # The 512-thread one-pair tiling: pivot and digit parity on a pair split
+    # over two of its workgroups, and the mid sizes that select it by default.
+    ctx = ctx_for(32)
+    u, v = wl.single_small(4)
+    ctx.set_option(mgcd.OPT_SINGLE_WORKGROUP, 512)
+    ctx.set_option(mgcd.OPT_SINGLE_SLOTS, 1)
+    try:
+        st, g, d, p = ctx.gcd_trace(u, v)
+        assert ctx.last_pair_stats()["tiling"] == (512, 1) and ctx.last_pair_stats()["parts"] == 2
+    finally:
+        ctx.set_option(mgcd.OPT_SINGLE_WORKGROUP, 0)
+        ctx.set_option(mgcd.OPT_SINGLE_SLOTS, 0)
+    nq = mgcd.num_moduli_needed(u.bit_length(), 32)
+    ost, od, ot = oracle.gcd_kernel(u, v, ctx.moduli()[:nq], 32)
+    assert st == mgcd.OK and ost == oracle.OK and p == ot and d == od and g == oracle.mpz_gcd(u, v)
+    for kib in (160, 288):
+        x, y = wl.pair(wl.Stream(71, kib), kib * 1024, kib * 100)
+        assert ctx.gcd(x, y) == oracle.mpz_gcd(x, y)
+        assert ctx.last_pair_stats()["tiling"] == (512, 1), kib
+
