@@ -13,6 +13,10 @@
 // harness wrote too.  The environment variable MGCD_L selects the modulus
 // width (22, 27 or 32; default 32).  Unlike the reference, the exit status
 // is 1 when any result differs from mpz_gcd.
+//
+// Built with -DMGCD_NO_GPU (bin/testmodgcd-nogpu) it times mpz_gcd alone on
+// the same pairs and needs neither a GPU nor the library, like the
+// reference's testmodgcd-nogpu (Makefile:61-62, testmodgcd.cpp:132-133).
 #include <sys/stat.h>
 #include <time.h>
 
@@ -60,6 +64,10 @@ int main(int argc, char* argv[]) {
   const int L = envL ? std::atoi(envL) : 32;
 
   std::cout << "GMP version " << gmp_version << "." << std::endl;
+#if defined(MGCD_NO_GPU)
+  (void)L;
+  std::cout << "Executing tests only on CPU." << std::endl;
+#else
   long long ttime;
   try {
     GmpCuda::GmpCudaDevice::setDevice(0);
@@ -73,6 +81,7 @@ int main(int argc, char* argv[]) {
   std::cout << "Max grid size = " << dev.getMaxGridSize() << std::endl
             << "Modulus width L = " << L << ", moduli available = " << dev.getNumModuli() << std::endl
             << "Initialization time: " << ttime / 1e6 << " ms." << std::endl;
+#endif
 
   mpz_t u, v, g, mod_g;
   mpz_inits(u, v, g, mod_g, NULL);
@@ -82,7 +91,10 @@ int main(int argc, char* argv[]) {
   do {
     std::cout << "***************************** Input size = " << num_bits << " (" << num_bits / 1024.0
               << " Kibit); gcd bits = " << num_g_bits << " *****************************" << std::endl;
-    long long atime = 0, mtime = 0;
+    long long mtime = 0;
+#if !defined(MGCD_NO_GPU)
+    long long atime = 0;
+#endif
     int numErrs = 0;
     FILE* pFile = nullptr;
     bool newFile = false;
@@ -101,7 +113,9 @@ int main(int argc, char* argv[]) {
       }
       if (!pFile) { std::cerr << "Error opening " << file << std::endl; return 4; }
     }
+#if !defined(MGCD_NO_GPU)
     bool warmup = true;
+#endif
     for (int i = 0; i < NUM_REPS; ++i) {
       if (random || newFile) {
         drawPair(u, v, g, state, num_bits, num_g_bits);
@@ -119,6 +133,7 @@ int main(int argc, char* argv[]) {
       mtime -= monotonicNs();
       mpz_gcd(g, u, v);
       mtime += monotonicNs();
+#if !defined(MGCD_NO_GPU)
       try {
         if (warmup) { dev.gcd(mod_g, u, v); warmup = false; }
         atime -= monotonicNs();
@@ -133,13 +148,18 @@ int main(int argc, char* argv[]) {
         gmp_printf("GMP_GCD = %Zx\nMOD_GCD = %Zx, size = %zu\n", g, mod_g, mpz_size(mod_g));
         numErrs += 1;
       }
+#endif
     }
     if (pFile) std::fclose(pFile);
     std::cout << std::fixed << std::setprecision(3);
+#if !defined(MGCD_NO_GPU)
     if (numErrs == 0) std::cout << "Modular gcd correct\n";
     std::cout << "MOD time(avg): " << atime / NUM_REPS / 1e6 << " ms\n";
+#endif
     std::cout << "GMP time     : " << mtime / NUM_REPS / 1e6 << " ms\n";
+#if !defined(MGCD_NO_GPU)
     std::cout << "MOD/GMP ratio: " << (double)atime / (double)mtime << std::endl;
+#endif
     totalErrs += numErrs;
     num_bits += increment;
   } while (increment && num_bits <= final_bits);

@@ -174,6 +174,29 @@ def test_reference_createModuli_matches_table_when_built():
     assert len(ref) == 65536 and mods[:65536].tolist() == ref
 
 
+def test_cpu_only_harness(tmp_path):
+    # bin/testmodgcd-nogpu times mpz_gcd alone (the reference's
+    # testmodgcd-nogpu, Makefile:61-62); its pair cache is the file the GPU
+    # harness replays.  When the reference's own CPU-only harness is built
+    # from source (oracle/_ref), both must draw and write the same pairs.
+    exe = os.path.join(PKG, "bin", "testmodgcd-nogpu")
+    a = tmp_path / "ours"
+    a.mkdir()
+    r = subprocess.run([exe, "3000", "500"], capture_output=True, text=True, timeout=60, cwd=a)
+    assert r.returncode == 0, r.stderr
+    assert "Executing tests only on CPU." in r.stdout and "GMP time" in r.stdout
+    assert "MOD time" not in r.stdout
+    ours = (a / "tests" / "3000-500").read_text()
+    assert len(ours.splitlines()) == 20
+    ref = os.path.join(ROOT, "oracle", "_ref", "testmodgcd-nogpu")
+    if not os.path.exists(ref):
+        pytest.skip("reference testmodgcd-nogpu not built here (oracle/_ref)")
+    b = tmp_path / "ref"
+    b.mkdir()
+    subprocess.run([ref, "3000", "500"], capture_output=True, text=True, timeout=60, cwd=b, check=True)
+    assert (b / "tests" / "3000-500").read_text() == ours
+
+
 def test_no_gpu_means_an_error_not_a_fallback(mgcd):
     n = ctypes.c_int(0)
     if mgcd.lib().mgcd_device_count(ctypes.byref(n)) == mgcd.OK and n.value > 0:
