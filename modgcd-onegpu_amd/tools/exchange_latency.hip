@@ -12,6 +12,13 @@
 //               and posts the minimum into a relay slot), and by XCD (a
 //               leader per XCD polls its XCD's slots, every workgroup polls
 //               the eight leaders' slots).  Every result is checked.
+//   XCD-local:  the same two exchanges through the XCD's L2 instead of past
+//               it: stores that write through the CU's L1 to the L2, polls
+//               with loads that skip the L1 (sc0) and are served by the L2.
+//               Only workgroups of one XCD share that L2, so the ping-pong
+//               runs between workgroups 0 and 8 (one XCD) and 0 and 1 (two
+//               XCDs: expected to give up), and the all-reduce among the
+//               workgroups of XCD 0 only.
 // Every wait is bounded (2^22 polls) and reported if it gives up.
 #include <hip/hip_runtime.h>
 
@@ -33,14 +40,23 @@ namespace {
 
 constexpr uint32_t kSpin = 1u << 22;
 
+// SYS 0: agent scope, 1: system scope, 2: XCD-local (through the L2).
 template <int SYS>
 __device__ __forceinline__ uint64_t ld(const uint64_t* p) {
-  if constexpr (SYS) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  else return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if constexpr (SYS == 2) {
+    uint64_t v;  // the load completes inside the asm block
+    asm volatile("global_load_dwordx2 %0, %1, off sc0\n\ts_waitcnt vmcnt(0)" : "=v"(v) : "v"(p) : "memory");
+    return v;
+  } else if constexpr (SYS == 1) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  } else {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 template <int SYS>
 __device__ __forceinline__ void st(uint64_t* p, uint64_t v) {
-  if constexpr (SYS) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  if constexpr (SYS == 2) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else if constexpr (SYS == 1) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint32_t xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20); }
@@ -77,10 +93,14 @@ __host__ __device__ inline uint64_t value_of(uint64_t wg, uint64_t r) { return (
 // MODE 0 flat, 1 relay, 2 by XCD.  slots [2][G], relay [2], xs [2][8][G], gs [2][8],
 // reg [9] counters; out[wg] = xor of the round minima (or ~0 after a give-up),
 // out[G + wg] real-time ticks, out[2G + wg] its XCD.
-template <int MODE>
+// SCOPE as for ld/st; with SCOPE 2 only every eighth workgroup (one XCD
+// under round-robin dispatch) takes part, as workgroup blockIdx / 8.
+template <int MODE, int SCOPE = 0>
 __global__ void allreduce(uint64_t* slots, uint64_t* relay, uint64_t* xs, uint64_t* gs, uint32_t* reg, int n,
                           uint64_t* out) {
-  const uint32_t G = gridDim.x, wg = blockIdx.x, lane = threadIdx.x;
+  constexpr uint32_t STRIDE = SCOPE == 2 ? 8 : 1;
+  if (blockIdx.x % STRIDE) return;
+  const uint32_t G = gridDim.x / STRIDE, wg = blockIdx.x / STRIDE, lane = threadIdx.x;
   __shared__ uint32_t s_li, s_nx[8], s_fail;
   const uint32_t x = xcc_id() & 7;
   if (lane == 0) {
@@ -112,7 +132,7 @@ __global__ void allreduce(uint64_t* slots, uint64_t* relay, uint64_t* xs, uint64
     const uint32_t row = r & 1;
     uint64_t m = ~0ull;
     auto wait_value = [&](const uint64_t* p) -> uint64_t {
-      uint64_t v = ld<0>(p);
+      uint64_t v = ld<SCOPE>(p);
       uint32_t s = 0;
       while ((v >> 32) != r) {
         if (++s > kSpin) {
@@ -120,16 +140,16 @@ __global__ void allreduce(uint64_t* slots, uint64_t* relay, uint64_t* xs, uint64
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        v = ld<0>(p);
+        v = ld<SCOPE>(p);
       }
       return v & 0xFFFFFFFFull;
     };
     if (MODE == 0 || MODE == 1) {
-      if (lane == 0) st<0>(slots + row * G + wg, tag | value_of(wg, r));
+      if (lane == 0) st<SCOPE>(slots + row * G + wg, tag | value_of(wg, r));
       if (MODE == 0 || wg == 0) {
         for (uint32_t j = lane; j < G; j += 64) m = std::min(m, wait_value(slots + row * G + j));
         for (int o = 32; o >= 1; o >>= 1) m = std::min(m, (uint64_t)__shfl_xor((unsigned long long)m, o));
-        if (MODE == 1 && lane == 0) st<0>(relay + row, tag | m);
+        if (MODE == 1 && lane == 0) st<SCOPE>(relay + row, tag | m);
       } else {
         m = wait_value(relay + row);
       }
@@ -186,8 +206,10 @@ int main() {
     if (pp(pingpong<0, 1>, "agent, s_sleep", 0, b)) return 1;
     if (pp(pingpong<0, 0>, "agent, no sleep", 0, b)) return 1;
     if (pp(pingpong<1, 0>, "system, no sleep", 0, b)) return 1;
+    if (pp(pingpong<2, 1>, "XCD-local, s_sleep", 0, b)) return 1;
+    if (pp(pingpong<2, 0>, "XCD-local, no sleep", 0, b)) return 1;
   }
-  auto ar = [&](auto kern, const char* name, int g) -> int {
+  auto ar = [&](auto kern, const char* name, int g, int stride = 1) -> int {
     CHECK(hipMemset(slots, 0, 2 * G * 8));
     CHECK(hipMemset(relay, 0, 16));
     CHECK(hipMemset(xs, 0, 2 * 8 * G * 8));
@@ -195,7 +217,7 @@ int main() {
     CHECK(hipMemset(reg, 0, 64));
     CHECK(hipMemset(out, 0, 3 * G * 8));
     const int nr = 5000;
-    hipLaunchKernelGGL(kern, dim3(g), dim3(64), 0, 0, slots, relay, xs, gs, reg, nr, out);
+    hipLaunchKernelGGL(kern, dim3(g * stride), dim3(64), 0, 0, slots, relay, xs, gs, reg, nr, out);
     CHECK(hipGetLastError());
     CHECK(hipDeviceSynchronize());
     CHECK(hipMemcpy(h.data(), out, 3 * G * 8, hipMemcpyDeviceToHost));
@@ -213,7 +235,7 @@ int main() {
       tmax = std::max(tmax, (double)h[g + w]);
       perx[h[2 * g + w] & 7]++;
     }
-    std::printf("all-reduce %-8s G = %3d: %7.1f ns per round, wrong %d, workgroups per XCD %d %d %d %d %d %d %d %d\n",
+    std::printf("all-reduce %-15s G = %3d: %7.1f ns per round, wrong %d, workgroups per XCD %d %d %d %d %d %d %d %d\n",
                 name, g, tmax * 10.0 / nr, bad, perx[0], perx[1], perx[2], perx[3], perx[4], perx[5], perx[6],
                 perx[7]);
     return 0;
@@ -222,6 +244,10 @@ int main() {
     if (ar(allreduce<0>, "flat", g)) return 1;
     if (ar(allreduce<1>, "relay", g)) return 1;
     if (ar(allreduce<2>, "by XCD", g)) return 1;
+  }
+  for (int g : {4, 10, 16, 32}) {
+    if (ar(allreduce<0, 2>, "XCD-local flat", g, 8)) return 1;
+    if (ar(allreduce<1, 2>, "XCD-local relay", g, 8)) return 1;
   }
   return 0;
 }
