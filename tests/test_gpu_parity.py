@@ -257,3 +257,41 @@ def test_one_pair_512_thread_tiling(ctx_for, mgcd, oracle, wl):
         assert ctx.gcd(x, y) == oracle.mpz_gcd(x, y)
         assert ctx.last_pair_stats()["tiling"] == (512, 1), kib
 
+
+
+def _structured_pairs(rnd):
+    # Structured and strongly unbalanced pairs, u >= v: powers of two,
+    # 2^a - 1 and 2^b - 1 (gcd 2^gcd(a, b) - 1), consecutive Fibonacci
+    # numbers (Euclid's worst case, coprime), v | u, v of 1 to 1000 bits
+    # beside a 20k-bit u (coprime and with a shared factor), u = v + 1, u = 3v.
+    fa, fb = 0, 1
+    for _ in range(28000):
+        fa, fb = fb, fa + fb
+    pairs = [(2**20000, 2**7000), (2**18000 - 1, 2**12000 - 1), (fb, fa)]
+    v = rnd.getrandbits(5000) | 1 << 4999
+    pairs.append((v * (rnd.getrandbits(12000) | 1 << 11999), v))
+    u = rnd.getrandbits(20000) | 1 << 19999
+    for vb in (1, 2, 31, 32, 33, 64, 1000):
+        vv = rnd.getrandbits(vb) | 1 << (vb - 1)
+        g = rnd.getrandbits(max(vb // 2, 2)) | 1
+        pairs += [(u, vv), (u * g, vv * g)]
+    vv = rnd.getrandbits(15000) | 1 << 14999
+    pairs += [(vv + 1, vv), (3 * vv, vv)]
+    return pairs
+
+
+@pytest.mark.parametrize("L", [32, 22])
+def test_structured_and_unbalanced_pairs(ctx_for, oracle, mgcd, L):
+    # Pivot and digit parity with the oracle for every pair alone, then the
+    # same pairs as one batch (default and group tilings) against math.gcd.
+    ctx = ctx_for(L)
+    pairs = _structured_pairs(random.Random(7))
+    for u, v in pairs:
+        _check_trace(ctx, oracle, mgcd, u, v, L)
+        assert ctx.gcd(u, v) == ctx.gcd(v, u) == math.gcd(u, v)
+    want = [math.gcd(u, v) for u, v in pairs]
+    for tiling in ({}, dict(workgroup=512, slots=2, group=16)):
+        b = mgcd.Batch.from_pairs(ctx, pairs + [(v, u) for u, v in pairs], **tiling)
+        b.run()
+        assert b.results() == want + want
+        b.close()
