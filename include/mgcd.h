@@ -66,7 +66,8 @@ int mgcd_set_device(int dev);
 /* Number of moduli the algorithm is started with for an input whose larger
  * operand has `bits` bits, at modulus width L (numModuliNeededFor,
  * GmpCudaDevice-gcd.cu:49-55): roundUp(ceil(C_L*bits/ln(bits)), 256), C_L =
- * 1.6 - 0.015 L, evaluated in float as the reference does. */
+ * 1.6 - 0.015 L, evaluated in float as the reference does.  bits < 2 is
+ * taken as 2 (the formula divides by ln(bits)); results saturate at 2^30. */
 int mgcd_num_moduli_needed(uint64_t bits, int L);
 
 /* Context: a device, a moduli table of width L in {22, 27, 32} uploaded to

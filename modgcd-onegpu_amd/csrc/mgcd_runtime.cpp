@@ -13,8 +13,9 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
-#include <deque>
 #include <cmath>
+#include <deque>
+#include <exception>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -69,6 +70,19 @@ thread_local std::string g_last_error;
       return MGCD_ERR_HIP;                                                            \
     }                                                                                 \
   } while (0)
+
+// No C++ exception crosses the C ABI: a host allocation that fails (or any
+// other exception the standard library raises, such as a thread that cannot
+// be started) becomes MGCD_ERR_NOMEM, with the detail in g_last_error.
+#define MGCD_CATCH                                   \
+  catch (const std::exception& e_) {                 \
+    g_last_error = std::string("host: ") + e_.what(); \
+    return MGCD_ERR_NOMEM;                           \
+  }                                                  \
+  catch (...) {                                      \
+    g_last_error = "host: unknown exception";        \
+    return MGCD_ERR_NOMEM;                           \
+  }
 
 // Grow-only device buffer.
 struct DevBuf {
@@ -607,14 +621,19 @@ int mgcd_set_device(int dev) {
 
 int mgcd_num_moduli_needed(uint64_t bits, int L) {
   // GmpCudaDevice-gcd.cu:49-55, in float as the reference computes it.
+  // ln(1) = 0 leaves the formula undefined below two bits: such inputs are
+  // sized as two-bit ones.  Counts beyond 2^30 saturate there (no table
+  // comes near it).
+  if (bits < 2) bits = 2;
   const float C_L = (float)(1.6 - 0.015 * L);
   float x = C_L * (float)bits;
   x = x / logf((float)bits);
+  if (!(x < 0x1p30f)) return 1 << 30;
   const int n = (int)ceilf(x);
   return ((n - 1) / 256 + 1) * 256;
 }
 
-int mgcd_ctx_create(mgcd_ctx** out, int device, int L, const char* table_path) {
+int mgcd_ctx_create(mgcd_ctx** out, int device, int L, const char* table_path) try {
   if (!out || (L != 22 && L != 27 && L != 32)) return MGCD_ERR_ARG;
   *out = nullptr;
   int n = 0;
@@ -706,7 +725,7 @@ int mgcd_ctx_create(mgcd_ctx** out, int device, int L, const char* table_path) {
   }
   *out = ctx;
   return MGCD_OK;
-}
+} MGCD_CATCH
 
 int mgcd_ctx_destroy(mgcd_ctx* ctx) {
   if (!ctx) return MGCD_OK;
@@ -1028,7 +1047,7 @@ extern "C" {
 
 int mgcd_gcd_limbs_trace(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint32_t* g,
                          size_t cap, size_t* ng, mgcd_pair_t* digits, size_t dcap, size_t* nd,
-                         mgcd_pair_t* pivots, size_t tcap, size_t* nt) {
+                         mgcd_pair_t* pivots, size_t tcap, size_t* nt) try {
   if (!ctx || (nu && !u) || (nv && !v) || !ng) return MGCD_ERR_ARG;
   HIP_TRY(hipSetDevice(ctx->device));
   nu = trim(u, nu);
@@ -1037,10 +1056,10 @@ int mgcd_gcd_limbs_trace(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint
   if (bits == 0) return MGCD_ERR_ARG;
   clear_last_pair(ctx);
   return gcd_single(ctx, u, nu, v, nv, moduli_for(ctx, bits), g, cap, ng, digits, dcap, nd, pivots, tcap, nt);
-}
+} MGCD_CATCH
 
 int mgcd_mixed_radix_to_limbs(mgcd_ctx* ctx, const mgcd_pair_t* digits, size_t k, uint32_t* g, size_t cap,
-                              size_t* ng) {
+                              size_t* ng) try {
   if (!ctx || (k && !digits) || !ng || k > 0xFFFFFFFFu) return MGCD_ERR_ARG;
   *ng = 0;
   if (k == 0) return MGCD_OK;
@@ -1055,10 +1074,10 @@ int mgcd_mixed_radix_to_limbs(mgcd_ctx* ctx, const mgcd_pair_t* digits, size_t k
   if (mag.size() > cap) return MGCD_ERR_CAPACITY;
   if (!mag.empty()) std::memcpy(g, mag.data(), mag.size() * 4);
   return MGCD_OK;
-}
+} MGCD_CATCH
 
 int mgcd_gcd_limbs(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint32_t* g,
-                   size_t cap, size_t* ng) {
+                   size_t cap, size_t* ng) try {
   if (!ctx || (nu && !u) || (nv && !v) || !ng) return MGCD_ERR_ARG;
   HIP_TRY(hipSetDevice(ctx->device));
   clear_last_pair(ctx);
@@ -1071,7 +1090,7 @@ int mgcd_gcd_limbs(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* 
   if (nv == 0) {  // gcd(u, 0) = |u|; the reference kernel cannot take v = 0
     *ng = nu;
     if (nu > cap) return MGCD_ERR_CAPACITY;
-    if (nu) std::memcpy(g, u, nu * 4);
+    if (nu) std::memmove(g, u, nu * 4);  // g may alias u
     return MGCD_OK;
   }
   // The reference's attempt first; where it would run out of moduli, retry
@@ -1083,7 +1102,7 @@ int mgcd_gcd_limbs(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* 
     if ((rc != MGCD_ERR_REDUX && rc != MGCD_ERR_RECOVERY) || !ctx->retry || ctx->force_nq || nq >= table) return rc;
     nq = std::min(table, ((nq + nq / 4 + 255) / 256) * 256);
   }
-}
+} MGCD_CATCH
 
 int mgcd_gcd_limbs_noalloc(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint32_t* g,
                            size_t cap, size_t* ng) {
@@ -1102,7 +1121,7 @@ int mgcd_gcd_limbs_noalloc(mgcd_ctx* ctx, const uint32_t* u, size_t nu, const ui
   if (nv == 0) {
     *ng = nu;
     if (nu > cap) return MGCD_ERR_CAPACITY;
-    if (nu) std::memcpy(g, u, nu * 4);
+    if (nu) std::memmove(g, u, nu * 4);  // g may alias u
     return MGCD_OK;
   }
   if (nu + nv > kScratchMaxLimbs) return MGCD_ERR_SCRATCH;
@@ -1382,7 +1401,7 @@ extern "C" {
 
 int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, const uint32_t* nu,
                       const uint64_t* v_off, const uint32_t* nv, size_t npairs, int opts_workgroup, int opts_slots,
-                      int opts_group, mgcd_batch** out) {
+                      int opts_group, mgcd_batch** out) try {
   if (!ctx || !out || (npairs && (!limbs || !u_off || !nu || !v_off || !nv))) return MGCD_ERR_ARG;
   *out = nullptr;
   HIP_TRY(hipSetDevice(ctx->device));
@@ -1478,10 +1497,20 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
       for (size_t i : byclass[c]) work[c] += (double)b->pd[b->kidx[i]].nq * (double)b->pd[b->kidx[i]].nq;
       if (!byclass[c].empty() && (cmax < 0 || work[c] > work[cmax])) cmax = c;
     }
+    // (Only where that tiling can split the class's largest pair over
+    // co-resident workgroups.)
+    auto fits = [&](int c, int vi) {
+      if (vi < 0) return false;
+      const KernelVariant& kv = kernel_variant(vi);
+      uint32_t nqmax = 0;
+      for (size_t i : byclass[c]) nqmax = std::max(nqmax, b->pd[b->kidx[i]].nq);
+      const uint32_t per_wg = (uint32_t)(kv.T * kv.K);
+      return (nqmax + per_wg - 1) / per_wg <= (uint32_t)ctx->max_resident[vi];
+    };
     if (cmax >= 0) pick(cmax);
     for (int c = 0; c < kNumClasses && rc == MGCD_OK; ++c) {
       if (byclass[c].empty() || c == cmax) continue;
-      if (work[c] < 0.1 * work[cmax]) class_variant[c] = class_variant[cmax];
+      if (work[c] < 0.1 * work[cmax] && fits(c, class_variant[cmax])) class_variant[c] = class_variant[cmax];
       else pick(c);
     }
     if (tuned_any) save_tune_cache(ctx);
@@ -1515,9 +1544,9 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
   }
   *out = b;
   return MGCD_OK;
-}
+} MGCD_CATCH
 
-int mgcd_batch_run(mgcd_batch* b) {
+int mgcd_batch_run(mgcd_batch* b) try {
   if (!b) return MGCD_ERR_ARG;
   mgcd_ctx* ctx = b->ctx;
   HIP_TRY(hipSetDevice(ctx->device));
@@ -1650,7 +1679,7 @@ int mgcd_batch_run(mgcd_batch* b) {
   }
   b->ran = true;
   return MGCD_OK;
-}
+} MGCD_CATCH
 
 int mgcd_batch_status(const mgcd_batch* b, size_t i) {
   if (!b || i >= b->npairs) return MGCD_ERR_ARG;
@@ -1676,7 +1705,7 @@ int mgcd_batch_timing(const mgcd_batch* b, double* device_s, double* host_s) {
   return MGCD_OK;
 }
 
-int mgcd_batch_pivots(const mgcd_batch* b, size_t i, mgcd_pair_t* pivots, size_t cap, size_t* n) {
+int mgcd_batch_pivots(const mgcd_batch* b, size_t i, mgcd_pair_t* pivots, size_t cap, size_t* n) try {
   if (!b || i >= b->npairs || !n || (cap && !pivots)) return MGCD_ERR_ARG;
   *n = 0;
   if (!b->ran || b->kidx[i] < 0) return MGCD_ERR_ARG;
@@ -1690,7 +1719,7 @@ int mgcd_batch_pivots(const mgcd_batch* b, size_t i, mgcd_pair_t* pivots, size_t
                       hipMemcpyDeviceToHost));
   }
   return MGCD_OK;
-}
+} MGCD_CATCH
 
 int mgcd_batch_pair_stats(const mgcd_batch* b, size_t i, uint32_t* nq, uint32_t* iters, uint32_t* digits) {
   if (!b || i >= b->npairs) return MGCD_ERR_ARG;
@@ -1763,7 +1792,7 @@ int mgcd_batch_destroy(mgcd_batch* b) {
 // Measures each candidate tiling on the given pairs (one warm-up and one
 // timed run of a batch with that tiling) and returns the fastest.
 int mgcd_autotune(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, const uint32_t* nu,
-                  const uint64_t* v_off, const uint32_t* nv, size_t npairs, int* workgroup, int* slots, int* group) {
+                  const uint64_t* v_off, const uint32_t* nv, size_t npairs, int* workgroup, int* slots, int* group) try {
   if (!ctx || !workgroup || !slots || !group || npairs == 0) return MGCD_ERR_ARG;
   double best_t = 0.0;
   int best = -1;
@@ -1795,10 +1824,10 @@ int mgcd_autotune(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_off, c
   *slots = kv.K;
   *group = kv.B;
   return MGCD_OK;
-}
+} MGCD_CATCH
 
 // ---------------------------------------------------------------------------
-int mgcd_certify_quasi_quotient(mgcd_ctx* ctx, int mode, int L, uint64_t* failures, uint64_t* checked) {
+int mgcd_certify_quasi_quotient(mgcd_ctx* ctx, int mode, int L, uint64_t* failures, uint64_t* checked) try {
   if (!ctx || !failures || !checked || mode < 0 || mode > 2 || (mode == 1 && (L < 2 || L > 32)))
     return MGCD_ERR_ARG;
   HIP_TRY(hipSetDevice(ctx->device));
@@ -1831,7 +1860,7 @@ int mgcd_certify_quasi_quotient(mgcd_ctx* ctx, int mode, int L, uint64_t* failur
   rc = run();
   cnt.release();
   return rc;
-}
+} MGCD_CATCH
 
 }  // extern "C"
 
@@ -2061,6 +2090,26 @@ void server_finish(mgcd_server* s, ServerSlot& sl, int launch_rc) {
   }
 }
 
+// The worker thread lets no exception out either: a launch whose host
+// bookkeeping throws answers its requests with MGCD_ERR_NOMEM.
+int server_launch_safe(mgcd_server* s, ServerSlot& sl) {
+  try {
+    return server_launch(s, sl);
+  } catch (...) {
+    return MGCD_ERR_NOMEM;
+  }
+}
+void server_finish_safe(mgcd_server* s, ServerSlot& sl, int launch_rc) {
+  try {
+    server_finish(s, sl, launch_rc);
+  } catch (...) {
+    for (ServerReq* r : sl.reqs) {
+      r->result.clear();
+      r->status = MGCD_ERR_NOMEM;
+    }
+  }
+}
+
 void server_loop(mgcd_server* s) {
   (void)hipSetDevice(s->ctx->device);
   std::unique_lock<std::mutex> lk(s->mu);
@@ -2070,7 +2119,7 @@ void server_loop(mgcd_server* s) {
     for (ServerSlot& sl : s->slots) {
       if (!sl.busy || hipEventQuery(sl.done_ev) != hipSuccess) continue;
       lk.unlock();
-      server_finish(s, sl, MGCD_OK);
+      server_finish_safe(s, sl, MGCD_OK);
       lk.lock();
       const auto now = std::chrono::steady_clock::now();
       if (s->trace)
@@ -2107,7 +2156,7 @@ void server_loop(mgcd_server* s) {
       sl.busy = true;
       sl.t_launch = std::chrono::steady_clock::now();
       lk.unlock();
-      const int rc = server_launch(s, sl);
+      const int rc = server_launch_safe(s, sl);
       lk.lock();
       if (s->trace)
         std::fprintf(stderr, "mgcd_server: slot %d (class %d) launch of %zu requests, variant %d, rc %d\n",
@@ -2115,7 +2164,7 @@ void server_loop(mgcd_server* s) {
       if (rc != MGCD_OK) {  // nothing is in flight on this slot: answer now
         (void)hipStreamSynchronize(sl.stream);
         lk.unlock();
-        server_finish(s, sl, rc);
+        server_finish_safe(s, sl, rc);
         lk.lock();
         const auto now = std::chrono::steady_clock::now();
         for (ServerReq* r : sl.reqs) {
@@ -2148,7 +2197,7 @@ void server_loop(mgcd_server* s) {
 
 extern "C" {
 
-int mgcd_server_create(mgcd_ctx* ctx, uint64_t budget_bytes, int slots, mgcd_server** out) {
+int mgcd_server_create(mgcd_ctx* ctx, uint64_t budget_bytes, int slots, mgcd_server** out) try {
   if (!ctx || !out || budget_bytes == 0 || slots < 1 || slots > 8) return MGCD_ERR_ARG;
   *out = nullptr;
   HIP_TRY(hipSetDevice(ctx->device));
@@ -2183,9 +2232,9 @@ int mgcd_server_create(mgcd_ctx* ctx, uint64_t budget_bytes, int slots, mgcd_ser
   s->worker = std::thread(server_loop, s);
   *out = s;
   return MGCD_OK;
-}
+} MGCD_CATCH
 
-int mgcd_server_submit(mgcd_server* s, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint64_t* ticket) {
+int mgcd_server_submit(mgcd_server* s, const uint32_t* u, size_t nu, const uint32_t* v, size_t nv, uint64_t* ticket) try {
   if (!s || !ticket || (nu && !u) || (nv && !v)) return MGCD_ERR_ARG;
   nu = trim(u, nu);
   nv = trim(v, nv);
@@ -2232,7 +2281,7 @@ int mgcd_server_submit(mgcd_server* s, const uint32_t* u, size_t nu, const uint3
   }
   s->cv.notify_all();
   return MGCD_OK;
-}
+} MGCD_CATCH
 
 int mgcd_server_wait(mgcd_server* s, uint64_t ticket, double timeout_s, uint32_t* g, size_t cap, size_t* ng,
                      double* latency_s) {
@@ -2257,7 +2306,7 @@ int mgcd_server_wait(mgcd_server* s, uint64_t ticket, double timeout_s, uint32_t
 }
 
 int mgcd_server_stats(mgcd_server* s, uint64_t* completed, double* p50_s, double* p99_s, double* gcds_per_s,
-                      uint64_t* peak_bytes) {
+                      uint64_t* peak_bytes) try {
   if (!s) return MGCD_ERR_ARG;
   std::lock_guard<std::mutex> lk(s->mu);
   std::vector<double> l = s->latencies;
@@ -2272,7 +2321,7 @@ int mgcd_server_stats(mgcd_server* s, uint64_t* completed, double* p50_s, double
   }
   if (peak_bytes) *peak_bytes = s->peak;
   return MGCD_OK;
-}
+} MGCD_CATCH
 
 int mgcd_server_destroy(mgcd_server* s) {
   if (!s) return MGCD_OK;

@@ -206,7 +206,8 @@ def test_no_gpu_means_an_error_not_a_fallback(mgcd):
 
 
 def test_sizing_matches_reference_formula_edges(mgcd):
-    # 1-bit operands: the reference divides by ln(1); the library sizes them
-    # as 2-bit operands when it runs them, and the raw formula is unchanged.
-    assert mgcd.num_moduli_needed(2, 32) == 256
+    # 0- and 1-bit operands: the reference divides by ln(1); the library
+    # sizes them as 2-bit operands.  Sizes beyond any table saturate at 2^30.
+    assert mgcd.num_moduli_needed(0, 32) == mgcd.num_moduli_needed(1, 32) == mgcd.num_moduli_needed(2, 32) == 256
+    assert mgcd.num_moduli_needed(2**64 - 1, 22) == mgcd.num_moduli_needed(2**40, 32) == 2**30
     assert mgcd.num_moduli_needed(1000000, 22) == 92160
