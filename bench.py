@@ -320,15 +320,16 @@ def run_skewed(args, m, w):
 
 def run_serve(args, m, w):
     """Poisson arrivals at five offered rates; sizes 70% 2k, 25% 50k, 5% 1M bits, g = 10% of the
-    length; 64 GB residue budget; three launch slots, one of them for the large requests."""
+    length; 64 GB residue budget; three launch slots, one of them for the large requests.  Then the
+    200/s arrivals again under a budget that binds (8 MiB: about three 1M-bit requests resident at
+    once), so that admission holds requests back."""
     import random
     import threading
     rates = [25, 50, 100, 200, 400]
     n = 300
-    out = []
     ctx = m.Context(32)
-    srv = m.Server(ctx, budget_bytes=64 << 30, slots=3)
-    for ri, rate in enumerate(rates):
+
+    def serve(srv, rate, ri):
         sizes = w.serve_sizes(n, seed=5 + ri)
         reqs = [w.serve_pair(100 + ri, i, nb) for i, nb in enumerate(sizes)]
         rnd = random.Random(ri)
@@ -369,18 +370,32 @@ def run_serve(args, m, w):
         per_size = {str(nb): {"n": sizes.count(nb), "p50_ms": pct([l for l, s2 in zip(lat, sizes) if s2 == nb], 0.5),
                               "p99_ms": pct([l for l, s2 in zip(lat, sizes) if s2 == nb], 0.99)}
                     for nb in sorted(set(sizes))}
-        out.append({"offered_per_s": rate, "completed": n, "achieved_per_s": round(n / span, 2),
-                    "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99), "by_bits": per_size,
-                    "peak_resident_mb_so_far": round(st["peak_bytes"] / 2**20, 1)})
-        log(f"rate {rate}/s: {out[-1]}")
+        rec = {"offered_per_s": rate, "completed": n, "achieved_per_s": round(n / span, 2),
+               "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99), "by_bits": per_size,
+               "peak_resident_mb_so_far": round(st["peak_bytes"] / 2**20, 3)}
+        log(f"rate {rate}/s: {rec}")
+        return rec, st["peak_bytes"]
+
+    out = []
+    srv = m.Server(ctx, budget_bytes=64 << 30, slots=3)
+    for ri, rate in enumerate(rates):
+        out.append(serve(srv, rate, ri)[0])
     srv.close()
+    tight = 8 << 20
+    srv = m.Server(ctx, budget_bytes=tight, slots=3)
+    rec, peak = serve(srv, 200, rates.index(200))
+    srv.close()
+    assert peak <= tight, f"resident bytes {peak} above the budget {tight}"
+    rec["budget_mb"] = tight / 2**20
     ctx.close()
     print(json.dumps({
         "metric": "serving: p50/p99 latency and GCD/s at offered rates; bit-exact",
         "unit": "ms", "n_gpus": 1, "higher_is_better": False,
         "config": {"workload": f"Poisson arrivals, {n} requests per rate, sizes 70% 2k / 25% 50k / 5% 1M bits, "
-                               "g = 10% of the length, 32-bit moduli, 64 GB budget, 3 launch slots (one for requests above 16,384 moduli)"},
+                               "g = 10% of the length, 32-bit moduli, 64 GB budget, 3 launch slots (one for requests above 16,384 moduli); "
+                               "then the 200/s arrivals under an 8 MiB budget"},
         "rates": out,
+        "binding_budget": rec,
     }), flush=True)
 
 
