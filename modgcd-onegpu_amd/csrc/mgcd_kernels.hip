@@ -1379,8 +1379,16 @@ __device__ __forceinline__ void start_ctx(Ctx& C, const KernelArgs& args, const 
   C.bar = args.bar + Gd.bar_off;
 }
 
+// Minimum waves per SIMD the compiler must leave room for.  Group tilings of
+// at most four waves per workgroup and one slot row otherwise spill into
+// AGPRs past 256 registers and run one wave per SIMD; held to two they fit
+// 255 VGPRs with at most 16 bytes of spill (measured at (256, 1, 16): 395k
+// against 234k GCD/s on 8,192 pairs of 8 kbit).
+constexpr int min_waves_per_simd(int T, int K, int B) { return (B > 1 && B <= 16 && K == 1 && T <= 256) ? 2 : 1; }
+
 template <int T, int K, int B, class A>
-__global__ __launch_bounds__(T) void gcd_kernel(KernelArgs args) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(min_waves_per_simd(T, K, B)))) void gcd_kernel(
+    KernelArgs args) {
   __shared__ Shared<T, K, B> S;
   Ctx C;
   const uint32_t xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);  // hwreg(HW_REG_XCC_ID, 0, 4)

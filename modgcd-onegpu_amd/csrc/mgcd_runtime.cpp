@@ -1199,8 +1199,21 @@ int class_of(uint32_t nq) {
 constexpr int kTuneCandidates[][3] = {{512, 2, 16}, {256, 2, 16}, {256, 1, 16}, {512, 1, 16},
                                      {256, 2, 8},  {256, 1, 8},  {256, 1, 32}, {128, 1, 16}};
 
-int default_variant(const mgcd_ctx* ctx, int cls) {
-  return cls == 0 ? find_kernel_variant(256, 1, 16, ctx->wide) : find_kernel_variant(512, 2, 16, ctx->wide);
+// The built-in tiling of a size class whose pairs have these N_Q: (512, 2,
+// 16) when its work items fill that tiling's co-resident grid (throughput:
+// 16 pairs share each inverse, one workgroup per CU, few parts), else
+// (256, 1, 16), whose four times smaller parts spread a small population
+// over more CUs (latency).  Measured at 8 kbit: 256 and 1,024 pairs 54k / 169k
+// against 37k / 143k GCD/s, 4,096 pairs 388k against 534k; at 30 kbit: 256
+// pairs 17.1k against 11.2k, 1,024 pairs 30.0k against 41.3k.  Pairs above
+// 16,384 moduli keep (512, 2, 16).
+int default_variant(const mgcd_ctx* ctx, int cls, const std::map<uint32_t, size_t>& nq_count) {
+  const int big = find_kernel_variant(512, 2, 16, ctx->wide);
+  const int small = find_kernel_variant(256, 1, 16, ctx->wide);
+  if (cls == kNumClasses - 1 || small < 0 || big < 0) return big;
+  size_t items = 0;
+  for (const auto& e : nq_count) items += (e.second + 15) / 16 * ((e.first + 1023) / 1024);
+  return items >= (size_t)ctx->max_resident[big] ? big : small;
 }
 
 }  // namespace
@@ -1484,7 +1497,9 @@ int mgcd_batch_create(mgcd_ctx* ctx, const uint32_t* limbs, const uint64_t* u_of
           tuned_any = true;
         }
       } else {
-        class_variant[c] = default_variant(ctx, c);
+        std::map<uint32_t, size_t> nq_count;
+        for (size_t i : byclass[c]) nq_count[b->pd[b->kidx[i]].nq]++;
+        class_variant[c] = default_variant(ctx, c, nq_count);
       }
     };
     // Elimination work of each class, about N_Q^2 per pair.  A class with

@@ -22,22 +22,42 @@ def _check(b, pairs):
 
 def test_size_classes_get_their_own_tilings(ctx_for, mgcd, wl):
     ctx = ctx_for(32)
-    # Two classes of comparable elimination work (about N_Q^2 per pair):
-    # small pairs (N_Q <= 1024) run with their own tiling and launch.
-    small = wl.headline(256, seed=31, nbits=5000, gbits=500)
-    mid = wl.headline(8, seed=35, nbits=30000, gbits=3000)
+    # Two classes of comparable elimination work (about N_Q^2 per pair), each
+    # with its own built-in tiling and launch: 4,096 small pairs (N_Q <= 1024)
+    # fill the device with (512, 2, 16) groups; 48 mid pairs (N_Q 3,328) do
+    # not, and take the smaller parts of (256, 1, 16).
+    small = wl.headline(4096, seed=31, nbits=5000, gbits=500)
+    mid = wl.headline(48, seed=35, nbits=30000, gbits=3000)
     assert mgcd.num_moduli_needed(5000, 32) <= 1024 < mgcd.num_moduli_needed(30000, 32) <= 4096
     pairs = small + mid
     b = mgcd.Batch.from_pairs(ctx, pairs)
     b.run()
     launches = b.launches()
     assert len(launches) == 2
-    assert launches[0]["tiling"] != launches[1]["tiling"]
+    assert [x["tiling"] for x in launches] == [(512, 2, 16), (256, 1, 16)]
     assert launches[0]["nq"][1] <= 1024 < launches[1]["nq"][0]
-    assert [x["pairs"] for x in launches] == [256, 8]
+    assert [x["pairs"] for x in launches] == [4096, 48]
     for x in launches:
         assert x["device_s"] > 0
     _check(b, pairs)
+    b.close()
+
+
+def test_built_in_tiling_follows_the_population(ctx_for, mgcd, wl):
+    # Small populations spread over the smaller parts of (256, 1, 16); one
+    # that fills the co-resident grid of (512, 2, 16) takes it.
+    ctx = ctx_for(32)
+    few = wl.headline(256, seed=38, nbits=8192, gbits=1024)
+    b = mgcd.Batch.from_pairs(ctx, few)
+    assert b.tiling() == (256, 1, 16)
+    b.run()
+    _check(b, few)
+    b.close()
+    many = few + wl.headline(3840, seed=39, nbits=8192, gbits=1024)
+    b = mgcd.Batch.from_pairs(ctx, many)
+    assert b.tiling() == (512, 2, 16)
+    b.run()
+    _check(b, many)
     b.close()
 
 
