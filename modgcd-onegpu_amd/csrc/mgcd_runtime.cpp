@@ -1963,6 +1963,10 @@ int server_launch(mgcd_server* s, ServerSlot& sl) {
   mgcd_ctx* ctx = s->ctx;
   std::vector<uint32_t> packed;
   sl.pd.clear();
+  // Kernel pair j is request order[j]; until the plan reorders them, the
+  // identity (a launch that fails before planning answers every request).
+  sl.order.resize(sl.reqs.size());
+  for (uint32_t j = 0; j < sl.order.size(); ++j) sl.order[j] = j;
   for (ServerReq* r : sl.reqs) {
     PairDesc p;
     std::memset(&p, 0, sizeof p);
@@ -2071,11 +2075,14 @@ void server_finish(mgcd_server* s, ServerSlot& sl, int launch_rc) {
   mgcd_ctx* ctx = s->ctx;
   // The digits of the finished pairs become |G| on the device (the product
   // tree batches use); only |G| comes back.
-  std::vector<int> sts(sl.pd.size(), launch_rc);
+  // A launch that failed answers every request (its descriptors may be
+  // incomplete); a finished one, kernel pair j = request order[j].
+  const bool ran = launch_rc == MGCD_OK && sl.pd.size() == sl.reqs.size() && sl.order.size() == sl.reqs.size();
+  std::vector<int> sts(sl.reqs.size(), ran ? MGCD_OK : (launch_rc != MGCD_OK ? launch_rc : MGCD_ERR_KERNEL));
   std::vector<uint64_t> offs;
   std::vector<uint32_t> ks;
   std::vector<std::vector<uint32_t>*> outs;
-  for (size_t j = 0; j < sl.pd.size() && launch_rc == MGCD_OK; ++j) {
+  for (size_t j = 0; j < sl.pd.size() && ran; ++j) {
     const PairOut& o = sl.h_out.as<PairOut>()[j];
     ServerReq* r = sl.reqs[sl.order[j]];
     sts[j] = o.status != KST_OK ? kst_to_status(o.status) : (o.ndig > sl.pd[j].nq ? MGCD_ERR_KERNEL : MGCD_OK);
@@ -2092,10 +2099,14 @@ void server_finish(mgcd_server* s, ServerSlot& sl, int launch_rc) {
       for (int& st : sts)
         if (st == MGCD_OK) st = crc;
   }
-  for (size_t j = 0; j < sl.pd.size(); ++j) {
-    ServerReq* r = sl.reqs[sl.order[j]];
+  for (size_t j = 0; j < sl.reqs.size(); ++j) {
+    ServerReq* r = sl.reqs[ran ? sl.order[j] : j];
     int st = sts[j];
-    if (st == MGCD_ERR_REDUX || st == MGCD_ERR_RECOVERY || st == MGCD_ERR_TIMEOUT) {
+    // Short of moduli, a timed-out wait, or a launch whose parts do not fit
+    // this slot's share of the device (requests of well over 1 Mbit, or many
+    // slots): the request alone through the one-pair path, whose tiling may
+    // use the whole device.
+    if (st == MGCD_ERR_REDUX || st == MGCD_ERR_RECOVERY || st == MGCD_ERR_TIMEOUT || st == MGCD_ERR_TOO_MANY_THREADS) {
       std::vector<uint32_t> g(r->v.size() + 1);
       size_t ng = 0;
       st = mgcd_gcd_limbs(ctx, r->u.data(), r->u.size(), r->v.data(), r->v.size(), g.data(), g.size(), &ng);

@@ -71,3 +71,19 @@ def test_budget_admission(mgcd, wl):
         srv.submit(*big)
     srv.close()
     ctx.close()
+
+
+def test_large_requests_beyond_a_slots_share(mgcd, wl):
+    # Eight slots leave each a sixteenth of the co-resident grid or less:
+    # eight 500k-bit requests (N_Q 42,752) submitted together split over more
+    # workgroups than the large-request slot's share when grouped.  Such a
+    # launch cannot be planned; its requests go through the one-pair path,
+    # which may use the whole device, and every result is still exact.
+    ctx = mgcd.Context(32)
+    srv = mgcd.Server(ctx, budget_bytes=1 << 30, slots=8)
+    reqs = [wl.serve_pair(43, i, 500000) for i in range(8)]
+    tickets = [srv.submit(*r) for r in reqs]
+    got = [srv.wait(t)[0] for t in tickets]
+    srv.close()
+    ctx.close()
+    assert got == [math.gcd(*r) for r in reqs]
