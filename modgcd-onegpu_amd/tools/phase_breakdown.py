@@ -6,11 +6,11 @@ cycles of each phase) and prints one JSON object: per workload the cycles of
 each phase per work item and per elimination round, converted to microseconds
 with the clock the real-time counter gives.
 
-    python modgcd-onegpu_amd/tools/phase_breakdown.py [--pairs N]
+    python modgcd-onegpu_amd/tools/phase_breakdown.py [--pairs N] [--tilings T,K,B ...] [--no-single]
 
-Workloads: 100k-bit pairs (10k-bit shared factor, L = 32) batched at tiling
-(512, 2, 16); one 100k-bit pair and one ~1M-bit pair (100k-bit shared
-factor) on the one-pair path.
+Workloads: 100k-bit pairs (10k-bit shared factor, L = 32) batched at each
+tiling given (default (512, 2, 16)); one 100k-bit pair and one ~1M-bit pair
+(100k-bit shared factor) on the one-pair path (skipped with --no-single).
 """
 import argparse
 import json
@@ -45,22 +45,26 @@ def summarize(c):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--pairs", type=int, default=2048)
+    ap.add_argument("--tilings", nargs="*", default=["512,2,16"])
+    ap.add_argument("--no-single", action="store_true")
     args = ap.parse_args()
     m = ge.package()
     w = ge.workload()
     res = {"library": os.path.relpath(m.LIB_PATH, ROOT)}
     ctx = m.Context(32)
     pairs = w.headline(args.pairs, seed=7)
-    b = m.Batch.from_pairs(ctx, pairs, workgroup=512, slots=2, group=16)
-    b.run()
-    ctx.phase_clocks(reset=True)
-    b.run()
-    dev = b.timing()[0]
-    res["batched_100k_512x2x16"] = {"pairs": args.pairs, "launch_ms": round(dev * 1e3, 2),
-                                    **summarize(ctx.phase_clocks(reset=True))}
-    b.close()
-    print("batched done", file=sys.stderr, flush=True)
-    for name, (u, v) in (("one_pair_100k", pairs[0]), ("one_pair_1M", w.single_large(0))):
+    for tl in args.tilings:
+        T, K, B = (int(x) for x in tl.split(","))
+        b = m.Batch.from_pairs(ctx, pairs, workgroup=T, slots=K, group=B)
+        b.run()
+        ctx.phase_clocks(reset=True)
+        b.run()
+        dev = b.timing()[0]
+        res[f"batched_100k_{T}x{K}x{B}"] = {"pairs": args.pairs, "launch_ms": round(dev * 1e3, 2),
+                                            **summarize(ctx.phase_clocks(reset=True))}
+        b.close()
+        print(f"batched {tl} done", file=sys.stderr, flush=True)
+    for name, (u, v) in (() if args.no_single else (("one_pair_100k", pairs[0]), ("one_pair_1M", w.single_large(0)))):
         ctx.gcd(u, v)
         ctx.phase_clocks(reset=True)
         ctx.gcd(u, v)
