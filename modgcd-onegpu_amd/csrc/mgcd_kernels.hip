@@ -143,6 +143,38 @@ __device__ __forceinline__ V load_after(LdsPtr<V>& p, int i, double after) {
   return p[i];
 }
 
+// Values every lane holds alike, moved into scalar registers.  process_group
+// is a called function whose arguments reach it through memory: reading
+// such a value once per work item and keeping it here keeps the per-round
+// paths (bookkeeping, exchange) free of loads of the kernel's arguments.
+__device__ __forceinline__ uint32_t uniform(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ uint64_t uniform(uint64_t x) {
+  return (uint64_t)uniform((uint32_t)x) | ((uint64_t)uniform((uint32_t)(x >> 32)) << 32);
+}
+template <class V>
+__device__ __forceinline__ V* uniform(V* p) {
+  return reinterpret_cast<V*>(uniform((uint64_t)reinterpret_cast<uintptr_t>(p)));
+}
+// A kernel argument array as a global pointer held in scalar registers.
+// (The host pass only parses device code; there the pointer stays generic,
+// since its C++ copies of structs cannot read an address-space-qualified
+// source.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class V>
+using GPtr = __attribute__((address_space(1))) V*;
+#else
+template <class V>
+using GPtr = V*;
+#endif
+template <class V>
+__device__ __forceinline__ GPtr<V> gptr(V* p) {
+  return (GPtr<V>)uniform(p);
+}
+// Exchange slots as global memory, so their atomics compile to global (not
+// flat) loads and stores.
+using SlotPtr = __attribute__((address_space(1))) unsigned long long*;
+__device__ __forceinline__ SlotPtr slots(unsigned long long* p) { return (SlotPtr)p; }
+
 // ---------------------------------------------------------------------------
 // Cross-lane minima of doubles.
 __device__ __forceinline__ uint32_t lo32(double x) { return (uint32_t)__builtin_bit_cast(uint64_t, x); }
@@ -300,6 +332,19 @@ struct Ctx {
   bool abort;                // a wait of this part timed out, or a peer's did: post ABORT from now on
 };
 
+__device__ __forceinline__ Ctx uniform_ctx(const Ctx& c) {
+  Ctx u;
+  u.G = uniform(c.G);
+  u.g = uniform(c.g);
+  u.rnd = uniform(c.rnd);
+  u.phase = uniform(c.phase);
+  u.bar = uniform(c.bar);
+  u.spin_limit = uniform(c.spin_limit);
+  u.relay = uniform((uint32_t)c.relay) != 0;
+  u.abort = uniform((uint32_t)c.abort) != 0;
+  return u;
+}
+
 // Cross-workgroup minima, called by wave 0 (all 64 lanes): lane b < B
 // brings this workgroup's value for pair b; lane l gets the group's minimum
 // for pair l % B, KNONE for every pair when a wait timed out or a peer
@@ -326,7 +371,7 @@ __device__ __forceinline__ void grid_post(double x, Ctx& C) {
   const uint32_t lane = __lane_id();
   C.rnd += 1;
   const uint64_t tag = round_tag(C);
-  unsigned long long* row = C.bar + (size_t)(C.rnd & 1) * C.G * B;
+  SlotPtr row = slots(C.bar) + (size_t)(C.rnd & 1) * C.G * B;
   if (lane < (uint32_t)B)
     __hip_atomic_store(row + (size_t)C.g * B + lane,
                        (unsigned long long)((tag << 52) | (C.abort ? ABORT : key_bits(x))), __ATOMIC_RELAXED,
@@ -341,8 +386,8 @@ __device__ double grid_collect(Ctx& C) {
   // phase means a peer has left the sequence, and counts as ABORT.
   auto ready = [&](uint64_t v) { return ((v >> 52) & 0x7FFu) == (tag & 0x7FFu); };
   auto aborted = [&](uint64_t v) { return (v >> 52) != tag || (v & PAYLOAD) == ABORT; };
-  unsigned long long* row = C.bar + (size_t)(C.rnd & 1) * C.G * B;
-  unsigned long long* relay = C.bar + (size_t)2 * C.G * B + (size_t)(C.rnd & 1) * B;
+  SlotPtr row = slots(C.bar) + (size_t)(C.rnd & 1) * C.G * B;
+  SlotPtr relay = slots(C.bar) + (size_t)2 * C.G * B + (size_t)(C.rnd & 1) * B;
   if (C.relay && C.g != 0) {
     uint64_t v = __hip_atomic_load(relay + lane % B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t spins = 0;
@@ -456,7 +501,7 @@ __device__ __forceinline__ double grid_min(double x, Ctx& C) {
 // unchanged, so every chain takes the same steps.  Limbs are staged
 // through LDS, TL / NOP per operand at a time, read from HBM coalesced.
 template <int T, int K, int NOP, int TL>
-__device__ void residues(const uint32_t* (&x)[NOP], const uint32_t (&n)[NOP], double (&r)[NOP][K],
+__device__ void residues(GPtr<const uint32_t> (&x)[NOP], const uint32_t (&n)[NOP], double (&r)[NOP][K],
                          const ModD (&M)[K], int rows, double* tile) {
   constexpr int TS = TL / NOP;
   uint32_t nmax = 0;
@@ -519,8 +564,17 @@ struct GroupRun {
   const GroupDesc Gd;
   SmallT& S;
   BigT& U;
-  Ctx& C;
+  Ctx C;  // this work item's exchange state, in scalar registers
   PhaseClock& clk;
+  // The kernel's arrays as global pointers in scalar registers (read once
+  // per work item): global, not flat, loads and stores with a scalar base.
+  const GPtr<const uint32_t> limbs;
+  const GPtr<const PairDesc> pairs;
+  const GPtr<const uint32_t> mod_u32;
+  const GPtr<const double> mod_d, mod_inv;
+  const GPtr<uint32_t> res;
+  const GPtr<mgcd_pair_t> trace, digits;
+  const GPtr<PairOut> out;
   const uint32_t tid, lane, wave, nq, npairs;
   // Part g owns moduli [g*nq/G, (g+1)*nq/G): sizes differ by at most one,
   // none is empty (G <= nq), none exceeds T*K (checked by the host).
@@ -537,7 +591,9 @@ struct GroupRun {
   PairState ps;
 
   __device__ __forceinline__ GroupRun(const KernelArgs& a, const GroupDesc& gd, SmallT& s, BigT& u, Ctx& c, PhaseClock& ck_)
-      : args(a), Gd(gd), S(s), U(u), C(c), clk(ck_), tid(threadIdx.x), lane(threadIdx.x & 63),
+      : args(a), Gd(gd), S(s), U(u), C(uniform_ctx(c)), clk(ck_), limbs(gptr(a.limbs)), pairs(gptr(a.pairs)),
+        mod_u32(gptr(a.mod_u32)), mod_d(gptr(a.mod_d)), mod_inv(gptr(a.mod_inv)), res(gptr(a.res)),
+        trace(gptr(a.trace)), digits(gptr(a.digits)), out(gptr(a.out)), tid(threadIdx.x), lane(threadIdx.x & 63),
         wave(threadIdx.x >> 6), nq(gd.nq), npairs(gd.count),
         base((uint32_t)(((uint64_t)c.g * gd.nq) / c.G)),
         cnt((uint32_t)(((uint64_t)(c.g + 1) * gd.nq) / c.G) - (uint32_t)(((uint64_t)c.g * gd.nq) / c.G)),
@@ -553,20 +609,20 @@ struct GroupRun {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const uint32_t j = (uint32_t)k * T + tid;
-      PM[k].m = j < pc ? args.mod_d[pb + j] : 3.0;
-      PM[k].minv = j < pc ? args.mod_inv[pb + j] : 1.0 / 3.0;
+      PM[k].m = j < pc ? mod_d[pb + j] : 3.0;
+      PM[k].minv = j < pc ? mod_inv[pb + j] : 1.0 / 3.0;
     }
     constexpr int PP = (B >= 2 && K <= 2) ? 2 : 1;
     for (uint32_t b0 = 0; b0 < npairs; b0 += PP) {
-      const uint32_t* x[2 * PP];
+      GPtr<const uint32_t> x[2 * PP];
       uint32_t nx[2 * PP];
       uint64_t ro[2 * PP];
 #pragma unroll
       for (int q = 0; q < PP; ++q) {
         const bool here = b0 + q < npairs;
-        const PairDesc& P = args.pairs[Gd.first + (here ? b0 + q : b0)];
-        x[2 * q] = args.limbs + P.u_off;
-        x[2 * q + 1] = args.limbs + P.v_off;
+        const PairDesc P = pairs[Gd.first + (here ? b0 + q : b0)];
+        x[2 * q] = limbs + P.u_off;
+        x[2 * q + 1] = limbs + P.v_off;
         nx[2 * q] = here ? P.nu : 0u;
         nx[2 * q + 1] = here ? P.nv : 0u;
         ro[2 * q] = P.res_off;
@@ -579,7 +635,7 @@ struct GroupRun {
         if (b0 + o / 2 >= npairs) break;
 #pragma unroll
         for (int k = 0; k < K; ++k)
-          if ((uint32_t)k * T + tid < pc) args.res[ro[o] + pb + k * T + tid] = (uint32_t)canon(r[o][k], PM[k]);
+          if ((uint32_t)k * T + tid < pc) res[ro[o] + pb + k * T + tid] = (uint32_t)canon(r[o][k], PM[k]);
       }
     }
   }
@@ -666,14 +722,14 @@ struct GroupRun {
     for (int k = 0; k < K; ++k) {
       const uint32_t j = (uint32_t)k * T + tid;
       if (j < cnt) {
-        M[k].m = args.mod_d[base + j];
-        M[k].minv = args.mod_inv[base + j];
+        M[k].m = mod_d[base + j];
+        M[k].minv = mod_inv[base + j];
       } else {
         // No modulus here: the lane's values are all sentinels, which enter
         // the elimination's products as they are, so it takes a table
         // modulus (a small one would make such a product 0 often).
-        M[k].m = args.mod_d[0];
-        M[k].minv = args.mod_inv[0];
+        M[k].m = mod_d[0];
+        M[k].minv = mod_inv[0];
       }
       ck[k] = 0.5 + 2.0 * (double)(IDX_MAX - (base + j));
     }
@@ -684,7 +740,7 @@ struct GroupRun {
   __device__ __forceinline__ void load_mods() {
     __syncthreads();
     if (lds_mods)
-      for (uint32_t i = tid; i < nq; i += T) U.u.mods[i] = args.mod_u32[i];
+      for (uint32_t i = tid; i < nq; i += T) U.u.mods[i] = mod_u32[i];
   }
 
   // Initial t = u/v (getGcdKernel.cu:369), one batched inverse per slot.
@@ -704,7 +760,7 @@ struct GroupRun {
 #pragma unroll
       for (int b = 0; b < B; ++b) {
         double v = 0.0;
-        if (here && (uint32_t)b < npairs) v = (double)args.res[args.pairs[Gd.first + b].res_off + nq + base + j];
+        if (here && (uint32_t)b < npairs) v = (double)res[pairs[Gd.first + b].res_off + nq + base + j];
         const double y = v != 0.0 ? v : 1.0;
         P[b] = b == 0 ? y : A::mulmod(P[b - 1], y, M[k]);
       }
@@ -713,9 +769,9 @@ struct GroupRun {
       for (int b = B - 1; b >= 0; --b) {
         double u = 0.0, v = 0.0;
         if (here && (uint32_t)b < npairs) {
-          const uint64_t ro = args.pairs[Gd.first + b].res_off + base + j;
-          u = (double)args.res[ro];
-          v = (double)args.res[ro + nq];
+          const uint64_t ro = pairs[Gd.first + b].res_off + base + j;
+          u = (double)res[ro];
+          v = (double)res[ro + nq];
         }
         const double inv = b == 0 ? r : A::mulmod(r, P[b - 1], M[k]);
         if (b > 0) r = A::mulmod(r, v != 0.0 ? v : 1.0, M[k]);
@@ -736,7 +792,7 @@ struct GroupRun {
     ps.trace_cap = 0;
     ps.trace_off = 0;
     if (wave == 0 && lane < npairs) {
-      const PairDesc& Pl = args.pairs[Gd.first + lane];
+      const PairDesc Pl = pairs[Gd.first + lane];
       ps.live = 1;
       ps.status = KST_OK;
       ps.remaining = (int32_t)nq;                   // totalModuliRemaining (:350)
@@ -798,18 +854,25 @@ struct GroupRun {
   // Records pivot (idx, val) as iteration niter of the pair: trace entry
   // (leader workgroup), and the next round's pivot in LDS.
   __device__ __forceinline__ void publish_pivot(uint32_t b, uint32_t idx, int32_t val) {
-    const uint32_t mu = lds_mods ? U.u.mods[idx] : args.mod_u32[idx];
+    // Address spaces explicit: an LDS read, or a global one (the group's
+    // table did not fit LDS), never a flat load through a merged pointer.
+    uint32_t mu;
+    if (lds_mods) mu = lds_ptr(U.u.mods)[idx];
+    else mu = mod_u32[idx];
+    S.bb[b] = (double)val;
+    S.pp[b] = (double)mu;
+    S.pidx[b] = idx;
+    // The trace store goes last: waits for mu must not include it (stores
+    // count in the same counter as global loads).
+    asm volatile("" ::: "memory");
     if (lead && ps.niter < ps.trace_cap) {
       mgcd_pair_t e;
       e.modulus = mu;
       e.value = val;
-      args.trace[ps.trace_off + ps.niter] = e;
+      trace[ps.trace_off + ps.niter] = e;
     }
     ps.niter++;
     ps.pval = val;
-    S.bb[b] = (double)val;
-    S.pp[b] = (double)mu;
-    S.pidx[b] = idx;
   }
 
   // Wave 0, after a round's minima kr (lane b: pair b's) are known: the
@@ -1085,11 +1148,11 @@ struct GroupRun {
           const uint32_t idx = base + (ent & 0x7FFu);
           eb[e] = ent >> 11;
           eR[e] = S.ntrace[eb[e]];
-          RM[e].m = args.mod_d[idx];
-          RM[e].minv = args.mod_inv[idx];
-          ro[e] = args.pairs[Gd.first + eb[e]].res_off + idx;
-          ra[e] = (double)args.res[ro[e]];
-          rc[e] = (double)args.res[ro[e] + nq];
+          RM[e].m = mod_d[idx];
+          RM[e].minv = mod_inv[idx];
+          ro[e] = pairs[Gd.first + eb[e]].res_off + idx;
+          ra[e] = (double)res[ro[e]];
+          rc[e] = (double)res[ro[e] + nq];
         }
       }
       // Entries in use by this wave (uniform).
@@ -1104,7 +1167,7 @@ struct GroupRun {
           const uint32_t b = j / len, jj = j - b * len;
           double bv = 0.0, pv = 1.0;
           if (S.status[b] == KST_OK && c0 + jj < S.ntrace[b]) {
-            const mgcd_pair_t tr = args.trace[args.pairs[Gd.first + b].trace_off + c0 + jj];
+            const mgcd_pair_t tr = trace[pairs[Gd.first + b].trace_off + c0 + jj];
             bv = (double)tr.value;
             pv = (double)tr.modulus;
           }
@@ -1136,8 +1199,8 @@ struct GroupRun {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         if (e0 + (uint32_t)e * T + tid < nsurv) {
-          args.res[ro[e]] = (uint32_t)canon(ra[e], RM[e]);
-          args.res[ro[e] + nq] = (uint32_t)canon(rs[e], RM[e]);
+          res[ro[e]] = (uint32_t)canon(ra[e], RM[e]);
+          res[ro[e] + nq] = (uint32_t)canon(rs[e], RM[e]);
         }
       }
     }
@@ -1160,7 +1223,7 @@ struct GroupRun {
         const uint32_t b = c0 + j;
         am[j] = 0;
         const bool run = b < npairs && S.status[b] == KST_OK;
-        const uint64_t ro = run ? args.pairs[Gd.first + b].res_off : 0;
+        const uint64_t ro = run ? pairs[Gd.first + b].res_off : 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           ua[j][k] = 0.0;
@@ -1168,8 +1231,8 @@ struct GroupRun {
           if (run && ((S.actw[b][k][wave] >> lane) & 1)) {
             am[j] |= 1u << k;
             const uint32_t idx = base + (uint32_t)k * T + tid;
-            ua[j][k] = (double)args.res[ro + idx];
-            us[j][k] = (double)args.res[ro + nq + idx];
+            ua[j][k] = (double)res[ro + idx];
+            us[j][k] = (double)res[ro + nq + idx];
           }
         }
       }
@@ -1240,7 +1303,7 @@ struct GroupRun {
       // Publishes digit (idx, val) of pair b (this lane's) as the next update.
       auto set_digit = [&](uint32_t idx, int32_t val) {
         S.bb[lane - c0] = (double)val;
-        S.pp[lane - c0] = args.mod_d[idx];
+        S.pp[lane - c0] = mod_d[idx];
         S.pidx[lane - c0] = idx;
         ps.pval = val;
       };
@@ -1299,9 +1362,9 @@ struct GroupRun {
         if (mine && lane < npairs && ps.live) {
           if (lead) {
             mgcd_pair_t dp;
-            dp.modulus = args.mod_u32[S.pidx[lane - c0]];
+            dp.modulus = mod_u32[S.pidx[lane - c0]];
             dp.value = ps.pval;
-            args.digits[args.pairs[Gd.first + lane].dig_off + ps.ndig] = dp;
+            digits[pairs[Gd.first + lane].dig_off + ps.ndig] = dp;
           }
           ps.ndig++;
           ps.remaining -= 1;                                       // :428
@@ -1331,7 +1394,7 @@ struct GroupRun {
       o.ndig = ps.ndig;
       o.ntrace = ps.niter;
       o.pad = 0;
-      args.out[Gd.first + lane] = o;
+      out[Gd.first + lane] = o;
     }
     clk.lap(PH_RECOVERY);
   }
