@@ -87,12 +87,18 @@ __device__ __forceinline__ void set_prio(int p) {
 // calibrates the clock.  Without the macro every call compiles to nothing.
 enum : int {
   PH_RESIDUES = 0, PH_COMPUTE, PH_BARRIER, PH_EXCHANGE, PH_BOOK, PH_BARRIER2, PH_TAIL, PH_REPLAY, PH_RECOVERY,
-  PH_ROUNDS, PH_ITEMS, PH_REALTIME, PH_MEMTIME
+  PH_ROUNDS, PH_ITEMS, PH_REALTIME, PH_MEMTIME,
+  // one-pass collects (groups of up to 256 slots): shader cycles from issuing
+  // a round's slot loads to their return, the collects whose first loads
+  // found every slot posted, and the collects
+  PH_FIRST_LOAD, PH_COLLECT_READY, PH_COLLECTS
 };
-static_assert(PH_MEMTIME + 1 <= kPhaseClocks, "phase clock slots");
+static_assert(PH_COLLECTS + 1 <= kPhaseClocks, "phase clock slots");
 struct PhaseClock {
 #ifdef MGCD_PHASE_CLOCKS
   uint64_t acc[PH_ROUNDS] = {};
+  uint64_t extra[3] = {};  // PH_FIRST_LOAD .. PH_COLLECTS
+  __device__ void acc_extra(int p, uint64_t v) { extra[p - PH_FIRST_LOAD] += v; }
   uint64_t t = 0, t0 = 0, r0 = 0, rounds = 0;
   __device__ void start() {
     t = t0 = __builtin_amdgcn_s_memtime();
@@ -112,6 +118,7 @@ struct PhaseClock {
     atomicAdd(out + PH_ITEMS, 1ull);
     atomicAdd(out + PH_REALTIME, (unsigned long long)(r - r0));
     atomicAdd(out + PH_MEMTIME, (unsigned long long)(n - t0));
+    for (int p = 0; p < 3; ++p) atomicAdd(out + PH_FIRST_LOAD + p, (unsigned long long)extra[p]);
   }
 #else
   __device__ void start() {}
@@ -330,6 +337,9 @@ struct Ctx {
   uint32_t spin_limit;
   bool relay;                // exchange through part 0 (large G)
   bool abort;                // a wait of this part timed out, or a peer's did: post ABORT from now on
+#ifdef MGCD_PHASE_CLOCKS
+  uint64_t first_load = 0, collect_ready = 0, collects = 0;  // see PH_FIRST_LOAD
+#endif
 };
 
 __device__ __forceinline__ Ctx uniform_ctx(const Ctx& c) {
@@ -456,11 +466,27 @@ __device__ double grid_collect(Ctx& C) {
   } else {
     const uint32_t j0 = lane;
     uint64_t v[PL];
+#ifdef MGCD_PHASE_CLOCKS
+    const uint64_t tl0 = __builtin_amdgcn_s_memtime();
+#endif
 #pragma unroll
     for (int q = 0; q < PL; ++q) {
       const uint32_t j = j0 + 64 * q;
       v[q] = j < nslots ? __hip_atomic_load(row + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag << 52) | NEUTRAL;
     }
+#ifdef MGCD_PHASE_CLOCKS
+    {
+      bool all = true;
+#pragma unroll
+      for (int q = 0; q < PL; ++q) all &= ready(v[q]);
+      C.first_load += __builtin_amdgcn_s_memtime() - tl0;
+      C.collect_ready += __ballot(!all) == 0;
+      C.collects += 1;
+    }
+#endif
+    // (Pipelined polls, several generations of loads in flight so that a
+    // late slot is seen sooner, measured slower: the extra poll traffic of
+    // every part lengthens every load's round trip.)
 #pragma unroll
     for (int q = 0; q < PL; ++q) {
       uint32_t spins = 0;
@@ -1414,6 +1440,11 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   g.finish_elimination();
   g.replay();
   g.recovery();
+#ifdef MGCD_PHASE_CLOCKS
+  clk.acc_extra(PH_FIRST_LOAD, g.C.first_load);
+  clk.acc_extra(PH_COLLECT_READY, g.C.collect_ready);
+  clk.acc_extra(PH_COLLECTS, g.C.collects);
+#endif
   clk.flush(args.clocks, threadIdx.x == 0);
 }
 

@@ -283,7 +283,8 @@ class Context:
         check(lib().mgcd_ctx_phase_clocks(self.h, out.ctypes.data_as(P64), 16, 1 if reset else 0),
               "mgcd_ctx_phase_clocks")
         d = {name: int(out[i]) for i, name in enumerate(self.PHASES)}
-        d.update(rounds=int(out[9]), items=int(out[10]), realtime_ticks=int(out[11]), memtime_cycles=int(out[12]))
+        d.update(rounds=int(out[9]), items=int(out[10]), realtime_ticks=int(out[11]), memtime_cycles=int(out[12]),
+                 first_load_cycles=int(out[13]), collects_ready=int(out[14]), collects=int(out[15]))
         return d
 
     def last_pair_stats(self):

@@ -39,6 +39,10 @@ def summarize(c):
            "share": {n: round(c[n] / tot, 4) for n in names},
            "per_round_us": {n: round(us(c[n] / rounds), 3) for n in names if n.startswith("elim_") and n != "elim_tail"}}
     out["per_round_us"]["total"] = round(sum(out["per_round_us"].values()), 3)
+    if c.get("collects"):
+        # one-pass collects: the slot loads' round trip, and how often they found every slot posted
+        out["collect_first_load_us"] = round(us(c["first_load_cycles"] / c["collects"]), 3)
+        out["collect_ready_at_first_load"] = round(c["collects_ready"] / c["collects"], 4)
     return out
 
 
