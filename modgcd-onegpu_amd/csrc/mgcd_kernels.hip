@@ -551,18 +551,23 @@ __device__ void residues(GPtr<const uint32_t> (&x)[NOP], const uint32_t (&n)[NOP
     __syncthreads();
     // Unrolled by eight with the group tilings' larger tiles (the limbs'
     // LDS reads issue under the previous steps' arithmetic; four measured
-    // 2 % slower, sixteen the same).
+    // 2 % slower, sixteen the same).  The number of slot rows in use is
+    // uniform: one branch here, not a select per step and row.
     constexpr int RU = TL > TILE ? 8 : 1;
+    auto steps = [&](auto all_rows) {
 #pragma unroll RU
-    for (uint32_t j = 0; j < len; ++j) {
+      for (uint32_t j = 0; j < len; ++j) {
 #pragma unroll
-      for (int o = 0; o < NOP; ++o) {
-        const double limb = tile[o * TS + j];
+        for (int o = 0; o < NOP; ++o) {
+          const double limb = tile[o * TS + j];
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-          if (k < rows) r[o][k] = horner_step(r[o][k], limb, M[k]);
+          for (int k = 0; k < K; ++k)
+            if (decltype(all_rows)::value || k < rows) r[o][k] = horner_step(r[o][k], limb, M[k]);
+        }
       }
-    }
+    };
+    if (rows == K) steps(std::true_type{});
+    else steps(std::false_type{});
   }
 }
 
@@ -1181,11 +1186,12 @@ struct GroupRun {
           rc[e] = (double)res[ro[e] + nq];
         }
       }
-      // Entries in use by this wave (uniform).
+      // Entries in use by this wave (uniform; in a scalar register).
       int ew = 0;
 #pragma unroll
       for (int e = 0; e < E; ++e)
         if (e0 + (uint32_t)e * T + wave * 64 < nsurv) ew = e + 1;
+      ew = __builtin_amdgcn_readfirstlane(ew);
       for (uint32_t c0 = 0; c0 < maxR; c0 += RC) {
         const uint32_t len = min((uint32_t)RC, maxR - c0);
         __syncthreads();
@@ -1201,31 +1207,47 @@ struct GroupRun {
           U.u.st.tile[2 * (b * RC + jj) + 1] = pv;
         }
         __syncthreads();
-        if (ew > 0) {
-          // Unrolled by eight in group tilings: the next pivots' LDS reads
-          // issue under the current ones' multiplications (the one-pair
-          // tiling measured slower unrolled).
+        // Unrolled by eight in group tilings: the next pivots' LDS reads
+        // issue under the current ones' multiplications (the one-pair tiling
+        // measured slower unrolled).  Every entry takes every step: past its
+        // pair's last pivot the staged step is (b, p') = (0, 1), which swaps
+        // a and c exactly (c' = a - 0 c, a' = c, s' = s), so no per-step test
+        // is needed; the parity is settled once, below.  The entries in use
+        // (uniform per wave) select one of E versions of the loop.
+        auto steps = [&](auto ne) {
+          constexpr int NE = decltype(ne)::value;
           constexpr int RU = B > 1 ? 8 : 1;
 #pragma unroll RU
           for (uint32_t j = 0; j < len; ++j) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-              if (e < ew && c0 + j < eR[e]) {
-                const double bv = U.u.st.tile[2 * (eb[e] * RC + j)], pv = U.u.st.tile[2 * (eb[e] * RC + j) + 1];
-                const double c2 = ra[e] - A::mulmod(bv, rc[e], RM[e]);  // |c2| <= m + 2^14
-                ra[e] = A::mulmod(rc[e], pv, RM[e]);
-                rs[e] = A::mulmod(rs[e], pv, RM[e]);
-                rc[e] = c2;
-              }
+            for (int e = 0; e < NE; ++e) {
+              const double bv = U.u.st.tile[2 * (eb[e] * RC + j)], pv = U.u.st.tile[2 * (eb[e] * RC + j) + 1];
+              const double c2 = ra[e] - A::mulmod(bv, rc[e], RM[e]);  // |c2| <= m + 2^14
+              ra[e] = A::mulmod(rc[e], pv, RM[e]);
+              rs[e] = A::mulmod(rs[e], pv, RM[e]);
+              rc[e] = c2;
+            }
+          }
+        };
+        if (ew == E) steps(std::integral_constant<int, E>{});
+        else if constexpr (E > 1) {
+          if (ew == 1) steps(std::integral_constant<int, 1>{});
+          else if constexpr (E > 2) {
+            if (ew == 2) steps(std::integral_constant<int, 2>{});
+            else if constexpr (E > 3) {
+              if (ew == 3) steps(std::integral_constant<int, 3>{});
             }
           }
         }
       }
-      // u = a/s on the survivors, kept as (a, s) in place of (u, v).
+      // u = a/s on the survivors, kept as (a, s) in place of (u, v).  After
+      // an odd number of padding swaps a sits in c (reduced: a swap moves
+      // the reduced a there).
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         if (e0 + (uint32_t)e * T + tid < nsurv) {
-          res[ro[e]] = (uint32_t)canon(ra[e], RM[e]);
+          const double af = ((maxR - eR[e]) & 1u) ? rc[e] : ra[e];
+          res[ro[e]] = (uint32_t)canon(af, RM[e]);
           res[ro[e] + nq] = (uint32_t)canon(rs[e], RM[e]);
         }
       }
