@@ -101,15 +101,18 @@ struct Arith32 {
   static __device__ double modinv(double y, ModD M) {
     double r0 = M.m, r1 = y, s0 = 0.0, s1 = 1.0, s;
     float f0 = (float)M.m, f1 = (float)y;
+    // The exit tests read the FP32 copies: for an integer r, |fl32(r)| > 1
+    // exactly when |r| > 1 (rounding is monotone and 1, 2 are floats), and
+    // the 32-bit compare issues at twice the FP64 rate.
     for (;;) {
       eea_step(r0, f0, s0, r1, f1, s1);
-      if (!(__builtin_fabs(r0) > 1.0)) break;
+      if (!(__builtin_fabsf(f0) > 1.0f)) break;
       eea_step(r1, f1, s1, r0, f0, s0);
-      if (!(__builtin_fabs(r1) > 1.0)) break;
+      if (!(__builtin_fabsf(f1) > 1.0f)) break;
     }
     // The remainder computed last is the one with |r| <= 1 (r0 if the loop
     // left after its first step); the other holds the one before it.
-    const bool last0 = !(__builtin_fabs(r0) > 1.0);
+    const bool last0 = !(__builtin_fabsf(f0) > 1.0f);
     const double rl = last0 ? r0 : r1, sl = last0 ? s0 : s1;
     const double rp = last0 ? r1 : r0, sp = last0 ? s1 : s0;
     s = rl != 0.0 ? rl * sl : rp * sp;
