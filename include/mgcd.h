@@ -106,8 +106,8 @@ enum {
    * on its first run and replays the graph afterwards.  0 (default):
    * cooperative launches. */
   MGCD_OPT_BATCH_GRAPH = 6,
-  /* Pairs split over at least this many workgroups (>= 2; default 48 for
-   * groups of pairs, 128 for one-pair launches) exchange their per-iteration
+  /* Pairs split over at least this many workgroups (>= 2; 0, the default:
+   * 48 for groups of pairs, 128 for one-pair launches) exchange their per-iteration
    * minima through part 0 (two hops, traffic linear in the parts) instead of
    * every part reading every part's slot (one hop, traffic quadratic in the
    * parts). */

@@ -576,14 +576,16 @@ int choose_single_variant(mgcd_ctx* ctx, uint32_t nq) {
       return vi;
   }
   if (nq <= 1024) return find_kernel_variant(1024, 1, 1, w);
-  // From 48 parts of 256 threads up to 128 parts of 512 threads the
+  // From 8 parts of 256 threads up to 128 parts of 512 threads the
   // 512-thread tiling (two waves per SIMD, half the parts, no relay) is
-  // faster (measured 10-15 % at 128-256 Kibit, 7-12 % at 352 Kibit); with
-  // more parts (1 Mbit) the 256-thread one, relayed.
+  // faster: measured 2-3 % at 16-48 Kibit, 4-7 % at 64-100 kbit, 10-15 % at
+  // 128-256 Kibit, 7-12 % at 352 Kibit (1,024 threads, a quarter of the
+  // parts, is 10-20 % slower throughout).  Below 8 parts (12 Kibit) the
+  // 256-thread tiling is 2 % faster; with more parts (1 Mbit) too, relayed.
   {
     const int v512 = find_kernel_variant(512, 1, 1, w);
     const uint32_t g256 = (nq + 255) / 256, g512 = (nq + 511) / 512;
-    if (v512 >= 0 && g256 >= kRelayGroups && g512 < kRelayOnePair && g512 <= (uint32_t)ctx->max_resident[v512])
+    if (v512 >= 0 && g256 >= 8 && g512 < kRelayOnePair && g512 <= (uint32_t)ctx->max_resident[v512])
       return v512;
   }
   for (int K : {1, 2, 4, 8}) {
@@ -780,8 +782,8 @@ int mgcd_ctx_set_option(mgcd_ctx* ctx, int option, int64_t value) {
       ctx->batch_graph = value != 0;
       return MGCD_OK;
     case MGCD_OPT_RELAY_PARTS:
-      if (value < 2) return MGCD_ERR_ARG;
-      ctx->relay_min_g = (uint32_t)std::min<int64_t>(value, 1 << 30);
+      if (value < 0 || value == 1) return MGCD_ERR_ARG;
+      ctx->relay_min_g = (uint32_t)std::min<int64_t>(value, 1 << 30);  // 0: the defaults
       return MGCD_OK;
     case MGCD_OPT_DEVICE_CONVERT:
       ctx->device_convert = value != 0;

@@ -99,6 +99,11 @@ def test_trace_multi_workgroup_relay(mgcd, oracle, wl):
         b.run()
         assert b.results() == [math.gcd(x, y) for x, y in pairs]
         b.close()
+        # 0 restores the default thresholds; 1 is not a threshold.
+        ctx.set_option(mgcd.OPT_RELAY_PARTS, 0)
+        with pytest.raises(mgcd.MgcdError):
+            ctx.set_option(mgcd.OPT_RELAY_PARTS, 1)
+        assert ctx.gcd(u, v) == math.gcd(u, v)
     finally:
         ctx.close()
 
@@ -252,7 +257,7 @@ def test_one_pair_512_thread_tiling(ctx_for, mgcd, oracle, wl):
     nq = mgcd.num_moduli_needed(u.bit_length(), 32)
     ost, od, ot = oracle.gcd_kernel(u, v, ctx.moduli()[:nq], 32)
     assert st == mgcd.OK and ost == oracle.OK and p == ot and d == od and g == oracle.mpz_gcd(u, v)
-    for kib in (160, 288):
+    for kib in (64, 160, 288):
         x, y = wl.pair(wl.Stream(71, kib), kib * 1024, kib * 100)
         assert ctx.gcd(x, y) == oracle.mpz_gcd(x, y)
         assert ctx.last_pair_stats()["tiling"] == (512, 1), kib
