@@ -177,6 +177,15 @@ template <class V>
 __device__ __forceinline__ GPtr<V> gptr(V* p) {
   return (GPtr<V>)uniform(p);
 }
+// A copy of a global pointer laundered into vector registers.  Where scalar
+// registers run short the compiler re-reads a kernel argument with a scalar
+// load, and waits for it, at each use; the laundered copy stays in place.
+template <class V>
+__device__ __forceinline__ GPtr<V> vptr(GPtr<V> p) {
+  uint64_t x = (uint64_t)(uintptr_t)p;
+  asm volatile("" : "+v"(x));
+  return (GPtr<V>)(uintptr_t)x;
+}
 // Exchange slots as global memory, so their atomics compile to global (not
 // flat) loads and stores.
 using SlotPtr = __attribute__((address_space(1))) unsigned long long*;
@@ -318,7 +327,8 @@ struct Big {
   union {
     struct {
       double tile[TL];           // limb tile / pivot tile (value, modulus)
-      uint16_t surv[B * T * K];  // survivors of the group: pair << 11 | local offset
+      uint16_t surv[B * T * K + 4 * B];  // survivors of the group: pair << 11 | local offset
+                                          // (each pair's run padded to four entries in group tilings)
     } st;
     uint32_t mods[MODS];         // (b): modulus of every index < N_Q
   } u;
@@ -606,6 +616,10 @@ struct GroupRun {
   const GPtr<uint32_t> res;
   const GPtr<mgcd_pair_t> trace, digits;
   const GPtr<PairOut> out;
+  // The two arrays the per-round bookkeeping reads and writes, in vector
+  // registers in one-pair tilings (short of scalar registers there).
+  const GPtr<const uint32_t> mod_u32_r;
+  const GPtr<mgcd_pair_t> trace_r;
   const uint32_t tid, lane, wave, nq, npairs;
   // Part g owns moduli [g*nq/G, (g+1)*nq/G): sizes differ by at most one,
   // none is empty (G <= nq), none exceeds T*K (checked by the host).
@@ -624,7 +638,9 @@ struct GroupRun {
   __device__ __forceinline__ GroupRun(const KernelArgs& a, const GroupDesc& gd, SmallT& s, BigT& u, Ctx& c, PhaseClock& ck_)
       : args(a), Gd(gd), S(s), U(u), C(uniform_ctx(c)), clk(ck_), limbs(gptr(a.limbs)), pairs(gptr(a.pairs)),
         mod_u32(gptr(a.mod_u32)), mod_d(gptr(a.mod_d)), mod_inv(gptr(a.mod_inv)), res(gptr(a.res)),
-        trace(gptr(a.trace)), digits(gptr(a.digits)), out(gptr(a.out)), tid(threadIdx.x), lane(threadIdx.x & 63),
+        trace(gptr(a.trace)), digits(gptr(a.digits)), out(gptr(a.out)),
+        mod_u32_r(B == 1 ? vptr(gptr(a.mod_u32)) : gptr(a.mod_u32)),
+        trace_r(B == 1 ? vptr(gptr(a.trace)) : gptr(a.trace)), tid(threadIdx.x), lane(threadIdx.x & 63),
         wave(threadIdx.x >> 6), nq(gd.nq), npairs(gd.count),
         base((uint32_t)(((uint64_t)c.g * gd.nq) / c.G)),
         cnt((uint32_t)(((uint64_t)(c.g + 1) * gd.nq) / c.G) - (uint32_t)(((uint64_t)c.g * gd.nq) / c.G)),
@@ -889,7 +905,7 @@ struct GroupRun {
     // table did not fit LDS), never a flat load through a merged pointer.
     uint32_t mu;
     if (lds_mods) mu = lds_ptr(U.u.mods)[idx];
-    else mu = mod_u32[idx];
+    else mu = mod_u32_r[idx];
     S.bb[b] = (double)val;
     S.pp[b] = (double)mu;
     S.pidx[b] = idx;
@@ -900,7 +916,7 @@ struct GroupRun {
       mgcd_pair_t e;
       e.modulus = mu;
       e.value = val;
-      trace[ps.trace_off + ps.niter] = e;
+      trace_r[ps.trace_off + ps.niter] = e;
     }
     ps.niter++;
     ps.pval = val;
@@ -1136,9 +1152,15 @@ struct GroupRun {
 
   // ---- (c1) replay of the pivots on the survivors ---------------------------
   // The survivors of every pair of the group form one compacted list
-  // (pair-major; entry = pair << 11 | slot offset); each lane replays up to
-  // four entries at a time, which may belong to different pairs: the pivots
-  // of all pairs are staged through LDS together, RC steps at a time.
+  // (pair-major; entry = pair << 11 | slot offset); the pivots of all pairs
+  // are staged through LDS together, RC steps at a time.  Group tilings
+  // (QUADS) pad each pair's run to a multiple of four entries and give a
+  // lane four consecutive entries, all of one pair, so that one LDS read of
+  // a step's pivot serves all four; one-pair tilings give lane t entries
+  // t, t + T, ... (up to K).
+  static constexpr bool QUADS = B >= 4;
+  static constexpr uint16_t PAD_ENTRY = 0xFFFF;  // a padding entry (pair 31, offset 2047: never a survivor)
+  static_assert(B < 32 || T * K < 2048, "padding entry collides with a survivor");
   __device__ __forceinline__ void replay() {
     uint32_t nsurv = 0, maxR = 0;
     for (uint32_t b = 0; b < npairs; ++b) {
@@ -1155,18 +1177,24 @@ struct GroupRun {
                 (uint16_t)((b << 11) | (uint32_t)(k * T + tid));
           nsurv += (uint32_t)__builtin_popcountll(word);
         }
+      if constexpr (QUADS) {
+        const uint32_t pad = (0u - nsurv) & 3u;
+        if (tid < pad) U.u.st.surv[nsurv + tid] = PAD_ENTRY;
+        nsurv += pad;
+      }
     }
     __syncthreads();
-    constexpr int E = B >= 4 ? 4 : K;  // entries per lane per round
+    constexpr int E = QUADS ? 4 : K;  // entries per lane per round
     constexpr int RC = BigT::TL / (2 * B);  // pivots per pair per staging
     for (uint32_t e0 = 0; e0 < nsurv; e0 += E * T) {
       ModD RM[E];
       double ra[E], rc[E], rs[E];
       uint32_t eb[E], eR[E];
       uint64_t ro[E];
+      bool ok[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const uint32_t j = e0 + (uint32_t)e * T + tid;
+        const uint32_t j = QUADS ? e0 + 4 * tid + (uint32_t)e : e0 + (uint32_t)e * T + tid;
         RM[e].m = 3.0;
         RM[e].minv = 1.0 / 3.0;
         ra[e] = rc[e] = 0.0;
@@ -1174,9 +1202,11 @@ struct GroupRun {
         ro[e] = 0;
         eb[e] = 0;
         eR[e] = 0;
-        if (j < nsurv) {
-          const uint32_t ent = U.u.st.surv[j];
+        ok[e] = false;
+        const uint32_t ent = j < nsurv ? (uint32_t)U.u.st.surv[j] : (uint32_t)PAD_ENTRY;
+        if (ent != PAD_ENTRY) {
           const uint32_t idx = base + (ent & 0x7FFu);
+          ok[e] = true;
           eb[e] = ent >> 11;
           eR[e] = S.ntrace[eb[e]];
           RM[e].m = mod_d[idx];
@@ -1186,11 +1216,16 @@ struct GroupRun {
           rc[e] = (double)res[ro[e] + nq];
         }
       }
-      // Entries in use by this wave (uniform; in a scalar register).
+      // Entries in use by this wave (uniform; in a scalar register): with
+      // QUADS all four or none (a quad's first entry is never padding).
       int ew = 0;
+      if constexpr (QUADS) {
+        if (e0 + 256 * wave < nsurv) ew = E;
+      } else {
 #pragma unroll
-      for (int e = 0; e < E; ++e)
-        if (e0 + (uint32_t)e * T + wave * 64 < nsurv) ew = e + 1;
+        for (int e = 0; e < E; ++e)
+          if (e0 + (uint32_t)e * T + wave * 64 < nsurv) ew = e + 1;
+      }
       ew = __builtin_amdgcn_readfirstlane(ew);
       for (uint32_t c0 = 0; c0 < maxR; c0 += RC) {
         const uint32_t len = min((uint32_t)RC, maxR - c0);
@@ -1219,9 +1254,15 @@ struct GroupRun {
           constexpr int RU = B > 1 ? 8 : 1;
 #pragma unroll RU
           for (uint32_t j = 0; j < len; ++j) {
+            double bq = 0.0, pq = 1.0;  // QUADS: the lane's pair's step, read once
+            if constexpr (QUADS) {
+              bq = U.u.st.tile[2 * (eb[0] * RC + j)];
+              pq = U.u.st.tile[2 * (eb[0] * RC + j) + 1];
+            }
 #pragma unroll
             for (int e = 0; e < NE; ++e) {
-              const double bv = U.u.st.tile[2 * (eb[e] * RC + j)], pv = U.u.st.tile[2 * (eb[e] * RC + j) + 1];
+              const double bv = QUADS ? bq : U.u.st.tile[2 * (eb[e] * RC + j)];
+              const double pv = QUADS ? pq : U.u.st.tile[2 * (eb[e] * RC + j) + 1];
               const double c2 = ra[e] - A::mulmod(bv, rc[e], RM[e]);  // |c2| <= m + 2^14
               ra[e] = A::mulmod(rc[e], pv, RM[e]);
               rs[e] = A::mulmod(rs[e], pv, RM[e]);
@@ -1230,7 +1271,7 @@ struct GroupRun {
           }
         };
         if (ew == E) steps(std::integral_constant<int, E>{});
-        else if constexpr (E > 1) {
+        else if constexpr (E > 1 && !QUADS) {
           if (ew == 1) steps(std::integral_constant<int, 1>{});
           else if constexpr (E > 2) {
             if (ew == 2) steps(std::integral_constant<int, 2>{});
@@ -1245,7 +1286,7 @@ struct GroupRun {
       // the reduced a there).
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        if (e0 + (uint32_t)e * T + tid < nsurv) {
+        if (ok[e]) {
           const double af = ((maxR - eR[e]) & 1u) ? rc[e] : ra[e];
           res[ro[e]] = (uint32_t)canon(af, RM[e]);
           res[ro[e] + nq] = (uint32_t)canon(rs[e], RM[e]);
