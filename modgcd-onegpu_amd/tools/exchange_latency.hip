@@ -40,22 +40,35 @@ namespace {
 
 constexpr uint32_t kSpin = 1u << 22;
 
-// SYS 0: agent scope, 1: system scope, 2: XCD-local (through the L2).
+// SYS 0: agent scope, 1: system scope, 2: XCD-local (through the L2), 3 and
+// 4: XCD-local with the CU's L1 invalidated before each poll (buffer_inv at
+// workgroup scope, sc0, or at agent scope, sc1) and a plain load, 5: agent
+// scope among the workgroups of one XCD (the comparison for 2-4).
 template <int SYS>
 __device__ __forceinline__ uint64_t ld(const uint64_t* p) {
   if constexpr (SYS == 2) {
     uint64_t v;  // the load completes inside the asm block
     asm volatile("global_load_dwordx2 %0, %1, off sc0\n\ts_waitcnt vmcnt(0)" : "=v"(v) : "v"(p) : "memory");
     return v;
+  } else if constexpr (SYS == 3) {
+    uint64_t v;
+    asm volatile("buffer_inv sc0\n\tglobal_load_dwordx2 %0, %1, off\n\ts_waitcnt vmcnt(0)" : "=v"(v) : "v"(p) : "memory");
+    return v;
+  } else if constexpr (SYS == 4) {
+    uint64_t v;
+    asm volatile("buffer_inv sc1\n\tglobal_load_dwordx2 %0, %1, off\n\ts_waitcnt vmcnt(0)" : "=v"(v) : "v"(p) : "memory");
+    return v;
   } else if constexpr (SYS == 1) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  } else if constexpr (SYS == 5) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 template <int SYS>
 __device__ __forceinline__ void st(uint64_t* p, uint64_t v) {
-  if constexpr (SYS == 2) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  if constexpr (SYS >= 2 && SYS <= 4) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   else if constexpr (SYS == 1) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -98,7 +111,7 @@ __host__ __device__ inline uint64_t value_of(uint64_t wg, uint64_t r) { return (
 template <int MODE, int SCOPE = 0>
 __global__ void allreduce(uint64_t* slots, uint64_t* relay, uint64_t* xs, uint64_t* gs, uint32_t* reg, int n,
                           uint64_t* out) {
-  constexpr uint32_t STRIDE = SCOPE == 2 ? 8 : 1;
+  constexpr uint32_t STRIDE = SCOPE >= 2 ? 8 : 1;
   if (blockIdx.x % STRIDE) return;
   const uint32_t G = gridDim.x / STRIDE, wg = blockIdx.x / STRIDE, lane = threadIdx.x;
   __shared__ uint32_t s_li, s_nx[8], s_fail;
@@ -208,6 +221,9 @@ int main() {
     if (pp(pingpong<1, 0>, "system, no sleep", 0, b)) return 1;
     if (pp(pingpong<2, 1>, "XCD-local, s_sleep", 0, b)) return 1;
     if (pp(pingpong<2, 0>, "XCD-local, no sleep", 0, b)) return 1;
+    if (pp(pingpong<3, 1>, "L1-inv sc0, s_sleep", 0, b)) return 1;
+    if (pp(pingpong<3, 0>, "L1-inv sc0, no sleep", 0, b)) return 1;
+    if (pp(pingpong<4, 1>, "L1-inv sc1, s_sleep", 0, b)) return 1;
   }
   auto ar = [&](auto kern, const char* name, int g, int stride = 1) -> int {
     CHECK(hipMemset(slots, 0, 2 * G * 8));
@@ -248,6 +264,11 @@ int main() {
   for (int g : {4, 10, 16, 32}) {
     if (ar(allreduce<0, 2>, "XCD-local flat", g, 8)) return 1;
     if (ar(allreduce<1, 2>, "XCD-local relay", g, 8)) return 1;
+    if (ar(allreduce<0, 3>, "L1-inv sc0 flat", g, 8)) return 1;
+    if (ar(allreduce<0, 4>, "L1-inv sc1 flat", g, 8)) return 1;
+    if (ar(allreduce<1, 4>, "L1-inv sc1 relay", g, 8)) return 1;
+    if (ar(allreduce<0, 5>, "one-XCD agent flat", g, 8)) return 1;
+    if (ar(allreduce<1, 5>, "one-XCD agent relay", g, 8)) return 1;
   }
   return 0;
 }
