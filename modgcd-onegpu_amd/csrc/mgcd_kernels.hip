@@ -1013,12 +1013,14 @@ struct GroupRun {
       double P[B], Y[B], key[B];
       bool pole_any = false;
       auto forward = [&](auto full) {
+        // Every y first: the pivot values' LDS reads all issue at the row's
+        // start (each address depends only on its own t, so the compiler
+        // cannot keep the values across rows), and the chain of products
+        // below then runs without waiting on LDS.
 #pragma unroll
         for (int b = 0; b < B; ++b) {
           const double tv = t[b][k];
-          // The pivot value is read two products ahead of its use, so the
-          // LDS latency overlaps the multiplications before it.
-          const double d = tv - load_after(bbp, b, b >= 3 ? P[b - 3] : tv);
+          const double d = tv - load_after(bbp, b, tv);
           double y = d;
           if constexpr (decltype(full)::value) {
             const bool fin = __builtin_fabs(tv) < TFIN;
@@ -1030,8 +1032,9 @@ struct GroupRun {
             y = __builtin_fabs(tv) < TFIN ? d : 1.0;
           }
           Y[b] = y;
-          P[b] = b == 0 ? y : A::mulmod(P[b - 1], y, M[k]);
         }
+#pragma unroll
+        for (int b = 0; b < B; ++b) P[b] = b == 0 ? Y[0] : A::mulmod(P[b - 1], Y[b], M[k]);
         if constexpr (B == 1) P[0] = reduce(P[0], M[k]);
       };
       // Issue priority falls as a wave moves through the round's phases
