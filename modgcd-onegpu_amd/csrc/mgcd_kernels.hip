@@ -437,14 +437,17 @@ __device__ double grid_collect(Ctx& C) {
   // to PL * 64 slots): a lane waits for its slots in turn and reloads only
   // the one it waits for (the joint form measured 0.5 % slower on 10-part
   // groups).  (Eight slots per lane, one pass for up to 512 slots, measured
-  // slower: 10 % on 45-part groups.)
+  // slower: 10 % on 45-part groups.)  One-pair tilings, short of nothing
+  // but the exchange, poll up to PJ = 8 slots per lane jointly: the relay
+  // leader of a 317-part pair reads all of them in one pass.
   constexpr int PL = 4;
+  constexpr int PJ = B == 1 ? 8 : PL;
   if (nslots > PL * 64) {
-    for (uint32_t j0 = lane; j0 < nslots && !late; j0 += PL * 64) {
-      uint64_t v[PL];
-      bool pend[PL];
+    for (uint32_t j0 = lane; j0 < nslots && !late; j0 += PJ * 64) {
+      uint64_t v[PJ];
+      bool pend[PJ];
 #pragma unroll
-      for (int q = 0; q < PL; ++q) {
+      for (int q = 0; q < PJ; ++q) {
         const uint32_t j = j0 + 64 * q;
         pend[q] = j < nslots;
         v[q] = pend[q] ? __hip_atomic_load(row + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag << 52) | NEUTRAL;
@@ -453,7 +456,7 @@ __device__ double grid_collect(Ctx& C) {
       for (;;) {
         bool any = false;
 #pragma unroll
-        for (int q = 0; q < PL; ++q) {
+        for (int q = 0; q < PJ; ++q) {
           pend[q] = pend[q] && !ready(v[q]);
           any |= pend[q];
         }
@@ -464,11 +467,11 @@ __device__ double grid_collect(Ctx& C) {
         }
         __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-        for (int q = 0; q < PL; ++q)
+        for (int q = 0; q < PJ; ++q)
           if (pend[q]) v[q] = __hip_atomic_load(row + j0 + 64 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
 #pragma unroll
-      for (int q = 0; q < PL; ++q) {
+      for (int q = 0; q < PJ; ++q) {
         none |= aborted(v[q]);
         m = __builtin_fmin(m, key_from_bits(v[q] & PAYLOAD));
       }
