@@ -103,15 +103,19 @@ struct Arith32 {
     float f0 = (float)M.m, f1 = (float)y;
     // The exit tests read the FP32 copies: for an integer r, |fl32(r)| > 1
     // exactly when |r| > 1 (rounding is monotone and 1, 2 are floats), and
-    // the 32-bit compare issues at twice the FP64 rate.
+    // the 32-bit compare issues at twice the FP64 rate.  One test per two
+    // steps: where the first step already reached |r0| <= 1, the second
+    // divides by r0 = +-1 (never 0: the remainders keep gcd(m, y) = 1, and
+    // the first step cannot give 0 since an odd m > 2^24 is not a float)
+    // and leaves r0, s0 alone, which the selection below then takes.
     for (;;) {
       eea_step(r0, f0, s0, r1, f1, s1);
-      if (!(__builtin_fabsf(f0) > 1.0f)) break;
       eea_step(r1, f1, s1, r0, f0, s0);
-      if (!(__builtin_fabsf(f1) > 1.0f)) break;
+      if (!(__builtin_fabsf(f0) > 1.0f) || !(__builtin_fabsf(f1) > 1.0f)) break;
     }
-    // The remainder computed last is the one with |r| <= 1 (r0 if the loop
-    // left after its first step); the other holds the one before it.
+    // The first remainder to reach |r| <= 1 is r0's if r0 is there (the
+    // step after it left r0 as it was), else r1's; the other holds the one
+    // before it.
     const bool last0 = !(__builtin_fabsf(f0) > 1.0f);
     const double rl = last0 ? r0 : r1, sl = last0 ? s0 : s1;
     const double rp = last0 ? r1 : r0, sp = last0 ? s1 : s0;
