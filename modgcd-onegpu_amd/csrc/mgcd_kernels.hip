@@ -398,9 +398,13 @@ __device__ __forceinline__ void grid_post(double x, Ctx& C) {
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// s_sleep units (64 cycles each) between a group part's post and its first
-// slot loads (see grid_collect).
+// s_sleep units (64 cycles each) between a part's post and its first slot
+// loads (see grid_collect): group tilings, one-pair tilings polling every
+// part, and one-pair parts waiting for the relay (two hops: the leader's
+// collect and its relay post).
 constexpr int kFirstPollSleep = 9;
+constexpr int kFirstPollSleepOnePair = 12;
+constexpr int kRelayPollSleep = 48;
 
 template <int B>
 __device__ double grid_collect(Ctx& C) {
@@ -413,6 +417,11 @@ __device__ double grid_collect(Ctx& C) {
   SlotPtr row = slots(C.bar) + (size_t)(C.rnd & 1) * C.G * B;
   SlotPtr relay = slots(C.bar) + (size_t)2 * C.G * B + (size_t)(C.rnd & 1) * B;
   if (C.relay && C.g != 0) {
+    // The relay slot cannot change before the leader has collected every
+    // part's post and posted the minimum: about 1.3 us before the first
+    // load (measured on one ~1M-bit pair, 317 parts: 253-263 ms without,
+    // 237-252 / 224-249 / 237-242 / 252 ms with 20 / 40 / 60 / 80 units).
+    __builtin_amdgcn_s_sleep(kRelayPollSleep);
     uint64_t v = __hip_atomic_load(relay + lane % B, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t spins = 0;
     bool late = false;
@@ -489,8 +498,10 @@ __device__ double grid_collect(Ctx& C) {
     // whole further round trip (past the L2, about 0.7 us) -- without the
     // wait 80 % of the rounds needed one.  Measured on 4,096 x 100 kbit:
     // 6,091 GCD/s without, 6,224 / 6,334 / 6,317 / 6,290 / 6,210 / 6,080 /
-    // 5,880 with 6 / 9 / 12 / 15 / 20 / 28 / 40 units.
+    // 5,880 with 6 / 9 / 12 / 15 / 20 / 28 / 40 units.  One-pair tilings
+    // (20 parts at 100 kbit): 20.3 ms without, 19.0-19.3 ms with 9 or 12.
     if constexpr (B > 1) __builtin_amdgcn_s_sleep(kFirstPollSleep);
+    else __builtin_amdgcn_s_sleep(kFirstPollSleepOnePair);
 #ifdef MGCD_PHASE_CLOCKS
     const uint64_t tl0 = __builtin_amdgcn_s_memtime();
 #endif
