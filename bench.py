@@ -263,7 +263,7 @@ def run_mixed(args, m, w):
     for i, (u, v) in enumerate(pairs):
         assert res[i] == math.gcd(u, v), f"pair {i}: result differs from math.gcd"
     launches = batch.launches()
-    mean1, p50, p99 = _one_per_launch(ctx, pairs, args.per_launch_sample * 4, 11)
+    one = _one_per_launch(ctx, pairs, args.per_launch_sample * 4, 11) if args.per_launch_sample > 0 else None
     value = len(pairs) / step_s
     print(json.dumps({
         "metric": "GCDs/sec batched vs one-per-launch, 4096 mixed-size pairs; bit-exact",
@@ -274,9 +274,10 @@ def run_mixed(args, m, w):
         "launches": [{"tiling": x["tiling"], "pairs": x["pairs"], "nq": x["nq"],
                       "ms": round(x["device_s"] * 1e3, 3)} for x in launches],
         "autotune_s": round(tune_s, 2),
-        "one_per_launch": {"gcds_per_s": round(1 / mean1, 3), "p50_ms": round(p50 * 1e3, 3),
-                           "p99_ms": round(p99 * 1e3, 3), "sample": args.per_launch_sample * 4},
-        "batched_vs_one_per_launch": round(value * mean1, 2),
+        "one_per_launch": ({"gcds_per_s": round(1 / one[0], 3), "p50_ms": round(one[1] * 1e3, 3),
+                            "p99_ms": round(one[2] * 1e3, 3), "sample": args.per_launch_sample * 4}
+                           if one else None),
+        "batched_vs_one_per_launch": round(value * one[0], 2) if one else None,
         "bit_exact": {"math_gcd_equal": len(pairs)},
     }), flush=True)
     batch.close()
