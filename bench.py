@@ -50,7 +50,7 @@ OPS_PER_RECOVERY = 15           # 2 mulmod + reduce per modulus-digit
 # HBM bytes per 100k-bit pair (10k-bit shared factor, L = 32) of the batched kernel, measured with
 # rocprofv3 FETCH_SIZE / WRITE_SIZE on the 4,096-pair launch and scaled by a copy kernel of known
 # size (FETCH_SIZE counts half the bytes, WRITE_SIZE all): 2 x 3,385,532 + 3,332,201 KiB per launch.
-HBM_BYTES_PER_HEADLINE_PAIR = (2 * 3365259 + 3332284) * 1024 / 4096
+HBM_BYTES_PER_HEADLINE_PAIR = (2 * 2047288 + 3332088) * 1024 / 4096
 
 
 def elim_ops_per_unit(L, B):
