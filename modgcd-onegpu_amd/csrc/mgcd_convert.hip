@@ -155,44 +155,162 @@ __global__ void mr_level_small(MrArgs a) {
   }
 }
 
-// Level j -> j + 1, one workgroup of 256 per task: columns in chunks of
-// 256, one per thread, then thread 0 resolves the chunk's carries.
-constexpr int MR_T = 256;
-__global__ __launch_bounds__(MR_T) void mr_level_large(MrArgs a) {
-  __shared__ int64_t colA[MR_T];
-  __shared__ uint64_t colB[MR_T];
-  uint32_t p, n;
-  find_task(a, blockIdx.x, p, n);
-  const MrPair P = a.pairs[p];
-  const Node lo = node_at(a, P, 2 * n), hi = node_at(a, P, 2 * n + 1);
-  const uint32_t w2 = 2u << a.j;
-  uint32_t* ov = a.out + P.buf_off + (uint64_t)n * (2u * w2 + 1u);
-  uint32_t* op = ov + w2 + 1u;
-  const uint32_t s = lo.s + hi.s;
-  const bool need_p = P.k > w2;
-  for (int which = 0; which < (need_p ? 2 : 1); ++which) {
-    const uint32_t ncol = which == 0 ? s + 1 : s;
-    uint32_t* o = which == 0 ? ov : op;
-    int64_t carry = 0;
-    uint64_t bprev = 0;
-    for (uint32_t c0 = 0; c0 < ncol; c0 += MR_T) {
-      const uint32_t c = c0 + threadIdx.x;
-      int64_t A = 0;
-      uint64_t B = 0;
-      if (c < ncol) {
-        if (which == 0) v_column(lo, hi, c, A, B);
-        else p_column(lo, hi, c, A, B);
-      }
-      colA[threadIdx.x] = A;
-      colB[threadIdx.x] = B;
+// Level j -> j + 1 for larger nodes: one wave per unit, a unit being 64
+// columns (a chunk) of one output node's V or P.  The wave stages the
+// operands through LDS in tiles of 64 limbs of P_lo and the 127 limbs of
+// the other factor those 64 columns meet, zero outside its range, so every
+// lane runs the same 64 products per tile with no bounds tests.  The chunk
+// is then resolved with no carry in; what it passes on goes to a carry
+// record that mr_fix applies (levels of one chunk per node need neither:
+// the carry out of a node's top limb is dropped, as every value is exact
+// modulo 2^(32 (s + 1))).
+__device__ __forceinline__ uint32_t factor_limb(const Node& hi, bool vpart, int64_t d) {
+  if (vpart) return d >= 0 && d <= (int64_t)hi.s ? v_limb(hi, (uint32_t)d) : 0u;
+  const uint32_t len = hi.s ? hi.s : 1u;  // an empty P_hi is 1
+  return d >= 0 && d < (int64_t)len ? p_limb(hi, (uint32_t)d) : 0u;
+}
+
+__global__ __launch_bounds__(kMrChunk) void mr_level_wave(MrArgs a, uint64_t units) {
+  __shared__ uint32_t lp[kMrChunk];
+  __shared__ uint32_t hv[2 * kMrChunk];
+  __shared__ int64_t colA[kMrChunk];
+  __shared__ uint64_t colB[kMrChunk];
+  const uint32_t lane = threadIdx.x;
+  const uint64_t cpt = mr_chunks(a.j);
+  const uint64_t w2 = 2ull << a.j;
+  for (uint64_t u = blockIdx.x; u < units; u += gridDim.x) {
+    const uint64_t task = u / (2 * cpt);
+    const bool vpart = ((u / cpt) & 1) == 0;
+    const uint64_t q = u % cpt;
+    uint32_t p, n;
+    find_task(a, (uint32_t)task, p, n);
+    const MrPair P = a.pairs[p];
+    const Node lo = node_at(a, P, 2 * n), hi = node_at(a, P, 2 * n + 1);
+    const uint64_t s = (uint64_t)lo.s + hi.s;
+    const bool need_p = P.k > w2;  // the output level is not this pair's root
+    const uint64_t ncol = vpart ? s + 1 : s;
+    const uint64_t c0 = q * kMrChunk;
+    if ((!vpart && !need_p) || c0 >= ncol) continue;  // uniform over the block
+    const uint64_t c = c0 + lane;
+    // Column c sums P_lo[i] * F[c - i] over i with 0 <= c - i <= hmax.
+    const int64_t hmax = vpart ? (int64_t)hi.s : (int64_t)(hi.s ? hi.s : 1u) - 1;
+    const uint64_t i_begin = (int64_t)c0 > hmax ? c0 - (uint64_t)hmax : 0;
+    const uint64_t i_end = min(c0 + kMrChunk, (uint64_t)lo.s);
+    uint64_t acc_lo = 0, acc_hi = 0;
+    for (uint64_t i0 = i_begin; i0 < i_end; i0 += kMrChunk) {
+      lp[lane] = i0 + lane < lo.s ? p_limb(lo, (uint32_t)(i0 + lane)) : 0u;
+      // hv[x] = F[c0 - i0 - 63 + x]: lane l, product ii reads hv[l + 63 - ii].
+      const int64_t d0 = (int64_t)c0 - (int64_t)i0 - (int64_t)(kMrChunk - 1);
+      hv[lane] = factor_limb(hi, vpart, d0 + lane);
+      hv[lane + kMrChunk] = lane + kMrChunk < 2 * kMrChunk - 1 ? factor_limb(hi, vpart, d0 + kMrChunk + lane) : 0u;
       __syncthreads();
-      if (threadIdx.x == 0) {
-        const uint32_t m = min((uint32_t)MR_T, ncol - c0);
-        for (uint32_t q = 0; q < m; ++q) {
-          const int64_t x = colA[q] + (int64_t)bprev + carry;
-          o[c0 + q] = (uint32_t)x;
-          carry = x >> 32;
-          bprev = colB[q];
+#pragma unroll 16
+      for (uint32_t ii = 0; ii < kMrChunk; ++ii) {
+        const uint64_t prod = (uint64_t)lp[ii] * (uint64_t)hv[lane + kMrChunk - 1 - ii];
+        acc_lo += (uint32_t)prod;
+        acc_hi += prod >> 32;
+      }
+      __syncthreads();
+    }
+    int64_t e = 0;
+    if (vpart && c < ncol) {
+      // + V_lo (sign-extended); V_hi < 0 was read as V_hi + 2^(32 (s_hi + 1)),
+      // so P_lo is subtracted once at limb s_hi + 1.
+      e = (int64_t)v_limb(lo, (uint32_t)c);
+      const bool neg_hi = (v_limb(hi, hi.s) >> 31) != 0;
+      if (neg_hi && c >= (uint64_t)hi.s + 1 && c - (hi.s + 1) < lo.s) e -= (int64_t)p_limb(lo, (uint32_t)(c - (hi.s + 1)));
+    }
+    colA[lane] = (int64_t)acc_lo + e;
+    colB[lane] = acc_hi;
+    __syncthreads();
+    const uint32_t m = (uint32_t)min((uint64_t)kMrChunk, ncol - c0);
+    uint32_t* o = a.out + P.buf_off + (uint64_t)n * (2 * w2 + 1) + (vpart ? 0 : w2 + 1) + c0;
+    if (lane == 0) {
+      int64_t carry = 0;
+      uint64_t bprev = 0;
+      for (uint32_t k = 0; k < m; ++k) {
+        const int64_t x = colA[k] + (int64_t)bprev + carry;
+        colA[k] = (int64_t)(uint32_t)x;
+        carry = x >> 32;
+        bprev = colB[k];
+      }
+      colB[0] = (uint64_t)(carry + (int64_t)bprev);
+    }
+    __syncthreads();
+    const uint32_t limb = (uint32_t)colA[lane];
+    if (lane < m) o[lane] = limb;
+    const bool ones = lane < 2 || lane >= m || limb == 0xFFFFFFFFu;
+    const bool zeros = lane < 2 || lane >= m || limb == 0u;
+    const bool all_ones = __all(ones), all_zeros = __all(zeros);
+    const uint32_t limb1 = __shfl(limb, 1);
+    if (lane == 0 && cpt > 1) {
+      MrCarry r;
+      r.out = (int64_t)colB[0];
+      r.limb0 = limb;
+      r.limb1 = m > 1 ? limb1 : 0u;
+      r.flags = (all_ones ? 1u : 0u) | (all_zeros ? 2u : 0u);
+      r.pad = 0;
+      a.carry[u] = r;
+    }
+    __syncthreads();
+  }
+}
+
+// Carries across the chunks of one output V or P (units of 2 per task):
+// chunk q's carry in is what chunk q - 1 passes on, plus the +-1 that a
+// carry in to q - 1 ripples out of it (only when its limbs above the two
+// lowest are all ones, or all zeros for a borrow).  Lane 0 runs that chain
+// over 64 chunks at a time; each lane then adds its chunk's carry in.
+__global__ __launch_bounds__(kMrChunk) void mr_fix(MrArgs a, uint64_t pairs_of_units) {
+  __shared__ MrCarry rec[kMrChunk];
+  __shared__ int64_t cin[kMrChunk];
+  const uint32_t lane = threadIdx.x;
+  const uint64_t cpt = mr_chunks(a.j);
+  const uint64_t w2 = 2ull << a.j;
+  for (uint64_t u2 = blockIdx.x; u2 < pairs_of_units; u2 += gridDim.x) {
+    const uint64_t task = u2 / 2;
+    const bool vpart = (u2 & 1) == 0;
+    uint32_t p, n;
+    find_task(a, (uint32_t)task, p, n);
+    const MrPair P = a.pairs[p];
+    const Node lo = node_at(a, P, 2 * n), hi = node_at(a, P, 2 * n + 1);
+    const uint64_t s = (uint64_t)lo.s + hi.s;
+    const bool need_p = P.k > w2;
+    const uint64_t ncol = vpart ? s + 1 : s;
+    if (!vpart && !need_p) continue;
+    const uint64_t nch = (ncol + kMrChunk - 1) / kMrChunk;
+    if (nch <= 1) continue;
+    const MrCarry* r = a.carry + (task * 2 + (vpart ? 0 : 1)) * cpt;
+    uint32_t* o = a.out + P.buf_off + (uint64_t)n * (2 * w2 + 1) + (vpart ? 0 : w2 + 1);
+    int64_t chain = 0;  // carry in of the next chunk to enter the chain
+    for (uint64_t qb = 0; qb < nch; qb += kMrChunk) {
+      const uint32_t nb = (uint32_t)min((uint64_t)kMrChunk, nch - qb);
+      if (lane < nb) rec[lane] = r[qb + lane];
+      __syncthreads();
+      if (lane == 0) {
+        for (uint32_t k = 0; k < nb; ++k) {
+          cin[k] = chain;
+          const int64_t x0 = (int64_t)rec[k].limb0 + chain;
+          const int64_t x1 = (int64_t)rec[k].limb1 + (x0 >> 32);
+          const int64_t up = x1 >> 32;  // -1, 0 or 1
+          int64_t rip = 0;
+          if (up == 1 && (rec[k].flags & 1u)) rip = 1;
+          if (up == -1 && (rec[k].flags & 2u)) rip = -1;
+          chain = rec[k].out + rip;
+        }
+      }
+      __syncthreads();
+      if (lane < nb && cin[lane] != 0) {
+        const uint64_t q = qb + lane;
+        const uint32_t m = (uint32_t)min((uint64_t)kMrChunk, ncol - q * kMrChunk);
+        uint32_t* oc = o + q * kMrChunk;
+        int64_t x = (int64_t)oc[0] + cin[lane];
+        oc[0] = (uint32_t)x;
+        int64_t cy = x >> 32;
+        for (uint32_t k = 1; cy != 0 && k < m; ++k) {
+          x = (int64_t)oc[k] + cy;
+          oc[k] = (uint32_t)x;
+          cy = x >> 32;
         }
       }
       __syncthreads();
@@ -201,41 +319,60 @@ __global__ __launch_bounds__(MR_T) void mr_level_large(MrArgs a) {
 }
 
 // |G| of every pair from its root (level root_level of the tree; the digit
-// itself when k == 1): magnitude limbs at out_off, trimmed length in lens.
-__global__ void mr_emit(MrArgs a, const uint32_t* buf0, const uint32_t* buf1, uint32_t* res, uint32_t* lens) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.npairs) return;
-  const MrPair P = a.pairs[p];
-  uint32_t* o = res + P.out_off;
-  if (P.k == 0) {
-    lens[p] = 0;
-    return;
-  }
-  const uint32_t* v = nullptr;
-  uint32_t tmp[2];
-  if (P.root_level == 0) {
-    const int32_t g = a.digits[P.dig_off].value;
-    tmp[0] = (uint32_t)g;
-    tmp[1] = g < 0 ? 0xFFFFFFFFu : 0u;
-    v = tmp;
-  } else {
-    v = ((P.root_level & 1) ? buf1 : buf0) + P.buf_off;
-  }
-  const bool neg = (v[P.k] >> 31) != 0;
-  uint32_t len = 0;
-  uint64_t c = 1;
-  for (uint32_t l = 0; l <= P.k; ++l) {
-    uint32_t x = v[l];
-    if (neg) {
-      const uint64_t y = (uint64_t)(~x) + c;
-      x = (uint32_t)y;
-      c = y >> 32;
+// itself when k == 1), one wave per pair: magnitude limbs at out_off,
+// trimmed length in lens.  For V < 0, |V| = ~V + 1, whose carry stops at
+// the lowest nonzero limb f: limbs below f stay 0, limb f is negated and
+// those above are complemented.
+__global__ __launch_bounds__(kMrChunk) void mr_emit(MrArgs a, const uint32_t* buf0, const uint32_t* buf1, uint32_t* res,
+                                                    uint32_t* lens) {
+  const uint32_t lane = threadIdx.x;
+  for (uint32_t p = blockIdx.x; p < a.npairs; p += gridDim.x) {
+    const MrPair P = a.pairs[p];
+    uint32_t* o = res + P.out_off;
+    if (P.k == 0) {
+      if (lane == 0) lens[p] = 0;
+      continue;
     }
-    o[l] = x;
-    if (x) len = l + 1;
+    if (P.root_level == 0) {
+      if (lane == 0) {
+        const int32_t g = a.digits[P.dig_off].value;
+        const uint32_t x = g < 0 ? 0u - (uint32_t)g : (uint32_t)g;
+        o[0] = x;
+        o[1] = 0;
+        lens[p] = x ? 1 : 0;
+      }
+      continue;
+    }
+    const uint32_t* v = ((P.root_level & 1) ? buf1 : buf0) + P.buf_off;
+    const uint64_t nl = (uint64_t)P.k + 1;
+    const bool neg = (v[P.k] >> 31) != 0;
+    uint64_t f = nl;
+    if (neg) {
+      for (uint64_t b = 0; b < nl; b += kMrChunk) {
+        const uint64_t mask = __ballot(b + lane < nl && v[b + lane] != 0u);
+        if (mask) {
+          f = b + (uint64_t)__ffsll((unsigned long long)mask) - 1;
+          break;
+        }
+      }
+    }
+    uint64_t len = 0;
+    for (uint64_t b = 0; b < nl; b += kMrChunk) {
+      const uint64_t l = b + lane;
+      uint32_t x = 0;
+      if (l < nl) {
+        x = v[l];
+        if (neg) x = l < f ? 0u : (l == f ? 0u - x : ~x);
+        o[l] = x;
+      }
+      const uint64_t mask = __ballot(x != 0u);
+      if (mask) len = b + 64 - (uint64_t)__clzll((unsigned long long)mask);
+    }
+    if (lane == 0) lens[p] = (uint32_t)len;
   }
-  lens[p] = len;
 }
+
+constexpr uint64_t kMrMaxGrid = 1u << 20;
 
 }  // namespace
 
@@ -243,8 +380,13 @@ hipError_t launch_mr_level(const MrArgs& a, uint32_t ntasks, hipStream_t s) {
   if (ntasks == 0) return hipSuccess;
   if (a.j < kMrSmallLevels) {
     hipLaunchKernelGGL(mr_level_small, dim3((ntasks + 255) / 256), dim3(256), 0, s, a);
-  } else {
-    hipLaunchKernelGGL(mr_level_large, dim3(ntasks), dim3(MR_T), 0, s, a);
+    return hipGetLastError();
+  }
+  const uint64_t units = mr_units(a.j, ntasks);
+  hipLaunchKernelGGL(mr_level_wave, dim3((uint32_t)min(units, kMrMaxGrid)), dim3(kMrChunk), 0, s, a, units);
+  if (mr_chunks(a.j) > 1) {
+    const uint64_t u2 = 2ull * ntasks;
+    hipLaunchKernelGGL(mr_fix, dim3((uint32_t)min(u2, kMrMaxGrid)), dim3(kMrChunk), 0, s, a, u2);
   }
   return hipGetLastError();
 }
@@ -252,7 +394,8 @@ hipError_t launch_mr_level(const MrArgs& a, uint32_t ntasks, hipStream_t s) {
 hipError_t launch_mr_emit(const MrArgs& a, const uint32_t* buf0, const uint32_t* buf1, uint32_t* res, uint32_t* lens,
                           hipStream_t s) {
   if (a.npairs == 0) return hipSuccess;
-  hipLaunchKernelGGL(mr_emit, dim3((a.npairs + 255) / 256), dim3(256), 0, s, a, buf0, buf1, res, lens);
+  hipLaunchKernelGGL(mr_emit, dim3((uint32_t)min((uint64_t)a.npairs, kMrMaxGrid)), dim3(kMrChunk), 0, s, a, buf0, buf1,
+                     res, lens);
   return hipGetLastError();
 }
 

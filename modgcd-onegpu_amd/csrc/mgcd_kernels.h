@@ -121,9 +121,39 @@ struct MrArgs {
   const uint32_t* cum;  // output nodes of pairs [0, p) at level j + 1; npairs + 1 entries
   const uint32_t* in;   // level j (j >= 1)
   uint32_t* out;        // level j + 1
+  struct MrCarry* carry;  // per-chunk carry records (levels of one wave per chunk)
 };
-// Levels below this (nodes of at most 16 digits) combine one node per thread.
-constexpr uint32_t kMrSmallLevels = 5;
+// Carry record of one 64-column chunk of an output node, resolved with no
+// carry in: what it passes on (it holds the last column's high-half sum, so
+// it can exceed 2^32), its two lowest limbs, and whether its limbs above
+// those are all ones (bit 0) or all zeros (bit 1).  A carry in c with
+// |c| < 2^63 leaves a carry of -1, 0 or 1 above the second limb, which
+// passes through the chunk only over all ones (1) or all zeros (-1).
+struct MrCarry {
+  int64_t out;
+  uint32_t limb0, limb1;
+  uint32_t flags, pad;
+};
+// Levels below this (nodes of at most 4 digits) combine one node per
+// thread; above it one wave computes 64 columns of a node's V or P.
+constexpr uint32_t kMrSmallLevels = 3;
+constexpr uint32_t kMrChunk = 64;
+// Chunks per output V or P at level j + 1 (V has at most 2^(j+1) + 1 limbs).
+constexpr uint64_t mr_chunks(uint32_t j) { return ((2ull << j) + kMrChunk) / kMrChunk; }
+// Work units of level j -> j + 1 over ntasks output nodes: V and P chunks
+// (one carry record each).
+constexpr uint64_t mr_units(uint32_t j, uint64_t ntasks) { return j < kMrSmallLevels ? 0 : ntasks * 2 * mr_chunks(j); }
+// Carry records a level of one pair of k digits needs at most (levels of
+// one chunk per node write none).
+constexpr uint64_t mr_max_records(uint64_t k) {
+  uint64_t m = 0;
+  for (uint32_t j = kMrSmallLevels; j < 63 && (1ull << j) < k; ++j) {
+    if (mr_chunks(j) < 2) continue;
+    const uint64_t u = mr_units(j, (k + (2ull << j) - 1) >> (j + 1));
+    m = u > m ? u : m;
+  }
+  return m;
+}
 hipError_t launch_mr_level(const MrArgs& a, uint32_t ntasks, hipStream_t s);
 // |G| per pair from its root: magnitude limbs at out_off, length in lens[p].
 hipError_t launch_mr_emit(const MrArgs& a, const uint32_t* buf0, const uint32_t* buf1, uint32_t* res, uint32_t* lens,

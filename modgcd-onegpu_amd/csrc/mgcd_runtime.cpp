@@ -158,17 +158,19 @@ int cmp_mag(const uint32_t* a, size_t na, const uint32_t* b, size_t nb) {
 
 // Buffers of the device conversion (grow-only).
 struct MrBufs {
-  DevBuf pairs, cum, buf0, buf1, res, lens;
+  DevBuf pairs, cum, buf0, buf1, res, lens, carry;
   std::vector<MrPair> h_pairs;
   std::vector<uint32_t> h_cum, h_res, h_lens;
   void release() {
-    for (DevBuf* d : {&pairs, &cum, &buf0, &buf1, &res, &lens}) d->release();
+    for (DevBuf* d : {&pairs, &cum, &buf0, &buf1, &res, &lens, &carry}) d->release();
   }
 };
 
-// One-pair calls convert fewer digits than this on the host (a launch
-// sequence costs more than a short Horner loop).
-constexpr uint32_t kMrHostDigits = 64;
+// One-pair calls convert this many digits or fewer on the host: the
+// device's launch sequence and copies cost about 0.27 ms whatever the
+// size, which the quadratic Horner loop reaches near 700 digits (measured
+// on MI355X: 0.16 ms at 313 digits, 0.36 ms at 1,001).
+constexpr uint32_t kMrHostDigits = 640;
 
 uint32_t ceil_log2(uint32_t k) {
   uint32_t j = 0;
@@ -210,10 +212,13 @@ int mr_convert(hipStream_t s, const mgcd_pair_t* d_digits, const std::vector<uin
       c[i + 1] = c[i] + t;
     }
   }
+  uint64_t records = 1;
+  for (uint32_t j = kMrSmallLevels; j < levels; ++j)
+    if (mr_chunks(j) > 1) records = std::max<uint64_t>(records, mr_units(j, mb.h_cum[(size_t)j * (n + 1) + n]));
   if (mb.pairs.ensure(n * sizeof(MrPair), retired) != hipSuccess ||
       mb.cum.ensure(mb.h_cum.size() * 4, retired) != hipSuccess || mb.buf0.ensure(buf * 4, retired) != hipSuccess ||
       mb.buf1.ensure(buf * 4, retired) != hipSuccess || mb.res.ensure(res * 4, retired) != hipSuccess ||
-      mb.lens.ensure(n * 4, retired) != hipSuccess)
+      mb.lens.ensure(n * 4, retired) != hipSuccess || mb.carry.ensure(records * sizeof(MrCarry), retired) != hipSuccess)
     return MGCD_ERR_NOMEM;
   HIP_TRY(hipMemcpyAsync(mb.pairs.p, mb.h_pairs.data(), n * sizeof(MrPair), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(mb.cum.p, mb.h_cum.data(), mb.h_cum.size() * 4, hipMemcpyHostToDevice, s));
@@ -222,6 +227,7 @@ int mr_convert(hipStream_t s, const mgcd_pair_t* d_digits, const std::vector<uin
   a.digits = d_digits;
   a.pairs = mb.pairs.as<MrPair>();
   a.npairs = n;
+  a.carry = mb.carry.as<MrCarry>();
   for (uint32_t j = 0; j < levels; ++j) {
     a.j = j;
     a.cum = mb.cum.as<uint32_t>() + (size_t)j * (n + 1);
@@ -324,7 +330,8 @@ constexpr uint32_t kScratchMaxLevels = 32;     // product-tree levels of the con
 constexpr size_t kScratchBytes = (size_t)2 << 20;
 
 struct ScratchLayout {
-  size_t limbs, pair, group, items, queue, bar, digits, trace, res, out, mrpair, mrcum, buf0, buf1, mres, mlen, end;
+  size_t limbs, pair, group, items, queue, bar, digits, trace, res, out, mrpair, mrcum, buf0, buf1, mres, mlen, mrcarry,
+      end;
 };
 constexpr size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 constexpr ScratchLayout scratch_layout() {
@@ -346,6 +353,7 @@ constexpr ScratchLayout scratch_layout() {
   l.buf1 = o;   o = al256(o + (8 * (size_t)kScratchMaxModuli + 8) * 4);
   l.mres = o;   o = al256(o + ((size_t)kScratchMaxModuli + 1) * 4);
   l.mlen = o;   o = al256(o + 4);
+  l.mrcarry = o; o = al256(o + std::max<uint64_t>(1, mr_max_records(kScratchMaxModuli)) * sizeof(MrCarry));
   l.end = o;
   return l;
 }
@@ -1020,6 +1028,7 @@ int gcd_single_noalloc(mgcd_ctx* ctx, unsigned char* base, const uint32_t* u, si
   a.digits = ka.digits;
   a.pairs = reinterpret_cast<const MrPair*>(base + kScratch.mrpair);
   a.npairs = 1;
+  a.carry = reinterpret_cast<MrCarry*>(base + kScratch.mrcarry);
   for (uint32_t j = 0; j < levels; ++j) {
     a.j = j;
     a.cum = reinterpret_cast<const uint32_t*>(base + kScratch.mrcum) + 2 * j;

@@ -4,8 +4,11 @@ replaces the host Horner loop, GmpCudaDevice-gcd.cu:113-129).
 The expected value is the digits' defining sum, G = sum g_i prod_{j<i} m_j,
 evaluated with Python integers: it pins the conversion independently of the
 GCD kernel.  Digit counts straddle every power of two the tree splits at,
-the one-thread-per-node / one-workgroup-per-node boundary (nodes of 16 and
-32 digits), and the size of a 1M-bit pair's GCD (about 3,100 digits)."""
+the one-thread-per-node / one-wave-per-chunk boundary (nodes of 4 and 8
+digits), the first level of more than one 64-limb chunk per node (64 and
+65 digits), the size of a 1M-bit pair's GCD (about 3,100 digits), and
+nodes of more than 64 chunks, whose carries are chained 64 chunks at a
+time (4,500 and 8,200 digits)."""
 import math
 import random
 
@@ -13,7 +16,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-KS = [1, 2, 3, 4, 5, 15, 16, 17, 31, 32, 33, 63, 64, 65, 127, 128, 129, 255, 256, 257, 1000, 1024, 1025, 3100]
+KS = [1, 2, 3, 4, 5, 7, 8, 9, 15, 16, 17, 31, 32, 33, 63, 64, 65, 127, 128, 129, 255, 256, 257, 1000, 1024, 1025, 3100,
+      4500, 8200]
 
 
 def _digits(moduli, k, rnd, mode):
@@ -67,6 +71,55 @@ def test_zero_and_sparse_digits(ctx_for):
         e = _digits(moduli, k, rnd, "random")
         e[-1] = (e[-1][0], 0)
         assert ctx.mixed_radix(e) == _value(e)
+
+
+def test_carries_ripple_across_chunks(ctx_for):
+    # g_0 = -1 and g_w = 1 make the node [0, 2w) equal P(0, w) - 1: every
+    # column above P_lo's top limb reads V_lo's sign extension (all ones), so
+    # the carry out of P_lo's top column ripples through all of them, across
+    # every chunk (and, at 8,200 digits, across a batch of 64 chunks) of that
+    # node; g_0 = 1, g_w = -1 runs the same with borrows.
+    ctx = ctx_for(32)
+    moduli = ctx.moduli()
+    for k in (300, 3100, 8200):
+        base = [(int(moduli[i]), 0) for i in range(k)]
+        w = 1
+        while w < k:
+            for a, b in ((-1, 1), (1, -1)):
+                e = list(base)
+                e[0] = (e[0][0], a)
+                e[w] = (e[w][0], b)
+                assert ctx.mixed_radix(e) == _value(e), (k, w, a, b)
+            w *= 2
+
+
+def _signed_digits(n, moduli):
+    # n in mixed radix over the moduli, digits in the kernel's toSigned range
+    out = []
+    i = 0
+    while n != 0:
+        m = int(moduli[i])
+        r = n % m
+        if r > (m - 3) // 2:
+            r -= m
+        out.append((m, r))
+        n = (n - r) // m
+        i += 1
+    return out
+
+
+def test_structured_values(ctx_for):
+    # Values whose limbs are long runs of ones or zeros (2^n - 1, 2^n, ...):
+    # a chunk's carry out, which holds a column's high-half sum and can pass
+    # 2^32, lands on such runs and must ripple through them, across chunks
+    # and batches of 64 chunks.  Either sign of G.
+    ctx = ctx_for(32)
+    moduli = ctx.moduli()
+    for n in (2100, 6000, 140000):  # 140,000 bits: about 4,380 digits, 69 chunks at the root
+        for x in (2**n - 1, 2**n, 2**n + 1, 2**n - 2 ** (n // 2), 3 * 2**n - 1):
+            for sign in (1, -1):
+                d = _signed_digits(sign * x, moduli)
+                assert ctx.mixed_radix(d) == x, (n, sign, len(d))
 
 
 @pytest.mark.parametrize("L", [22, 27])
