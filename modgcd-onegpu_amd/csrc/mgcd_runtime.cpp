@@ -159,10 +159,13 @@ int cmp_mag(const uint32_t* a, size_t na, const uint32_t* b, size_t nb) {
 // Buffers of the device conversion (grow-only).
 struct MrBufs {
   DevBuf pairs, cum, buf0, buf1, res, lens, carry;
+  PinnedBuf h_res, h_lens;  // |G| limbs and lengths come back through pinned memory
   std::vector<MrPair> h_pairs;
-  std::vector<uint32_t> h_cum, h_res, h_lens;
+  std::vector<uint32_t> h_cum;
   void release() {
     for (DevBuf* d : {&pairs, &cum, &buf0, &buf1, &res, &lens, &carry}) d->release();
+    h_res.release();
+    h_lens.release();
   }
 };
 
@@ -181,10 +184,11 @@ uint32_t ceil_log2(uint32_t k) {
 // |G| of every (dig_off, k) listed, from the digits in device memory at
 // d_digits, by the product tree of mgcd_convert.hip on stream s; waits for
 // the stream.  out[i] receives pair i's limbs.  Returns an MGCD_ status.
-// retired: as in DevBuf::ensure (callers whose device runs other streams' kernels).
+// retired, retired_host: as in DevBuf::ensure and PinnedBuf::ensure (callers whose device
+// runs other streams' kernels).
 int mr_convert(hipStream_t s, const mgcd_pair_t* d_digits, const std::vector<uint64_t>& dig_off,
                const std::vector<uint32_t>& ks, MrBufs& mb, std::vector<std::vector<uint32_t>*>& out,
-               std::vector<void*>* retired = nullptr) {
+               std::vector<void*>* retired = nullptr, std::vector<void*>* retired_host = nullptr) {
   const uint32_t n = (uint32_t)ks.size();
   if (n == 0) return MGCD_OK;
   mb.h_pairs.resize(n);
@@ -218,7 +222,8 @@ int mr_convert(hipStream_t s, const mgcd_pair_t* d_digits, const std::vector<uin
   if (mb.pairs.ensure(n * sizeof(MrPair), retired) != hipSuccess ||
       mb.cum.ensure(mb.h_cum.size() * 4, retired) != hipSuccess || mb.buf0.ensure(buf * 4, retired) != hipSuccess ||
       mb.buf1.ensure(buf * 4, retired) != hipSuccess || mb.res.ensure(res * 4, retired) != hipSuccess ||
-      mb.lens.ensure(n * 4, retired) != hipSuccess || mb.carry.ensure(records * sizeof(MrCarry), retired) != hipSuccess)
+      mb.lens.ensure(n * 4, retired) != hipSuccess || mb.carry.ensure(records * sizeof(MrCarry), retired) != hipSuccess ||
+      mb.h_res.ensure(res * 4, retired_host) != hipSuccess || mb.h_lens.ensure(n * 4, retired_host) != hipSuccess)
     return MGCD_ERR_NOMEM;
   HIP_TRY(hipMemcpyAsync(mb.pairs.p, mb.h_pairs.data(), n * sizeof(MrPair), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(mb.cum.p, mb.h_cum.data(), mb.h_cum.size() * 4, hipMemcpyHostToDevice, s));
@@ -236,14 +241,12 @@ int mr_convert(hipStream_t s, const mgcd_pair_t* d_digits, const std::vector<uin
     HIP_TRY(launch_mr_level(a, mb.h_cum[(size_t)j * (n + 1) + n], s));
   }
   HIP_TRY(launch_mr_emit(a, lv[0], lv[1], mb.res.as<uint32_t>(), mb.lens.as<uint32_t>(), s));
-  mb.h_res.resize(res);
-  mb.h_lens.resize(n);
-  HIP_TRY(hipMemcpyAsync(mb.h_res.data(), mb.res.p, res * 4, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(mb.h_lens.data(), mb.lens.p, n * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(mb.h_res.p, mb.res.p, res * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(mb.h_lens.p, mb.lens.p, n * 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   for (uint32_t i = 0; i < n; ++i) {
-    const uint32_t* r = mb.h_res.data() + mb.h_pairs[i].out_off;
-    out[i]->assign(r, r + mb.h_lens[i]);
+    const uint32_t* r = mb.h_res.as<uint32_t>() + mb.h_pairs[i].out_off;
+    out[i]->assign(r, r + mb.h_lens.as<uint32_t>()[i]);
   }
   return MGCD_OK;
 }
@@ -2105,7 +2108,8 @@ void server_finish(mgcd_server* s, ServerSlot& sl, int launch_rc) {
     }
   }
   if (!ks.empty()) {
-    const int crc = mr_convert(sl.stream, sl.digits.as<mgcd_pair_t>(), offs, ks, sl.mr, outs, &sl.retired_dev);
+    const int crc = mr_convert(sl.stream, sl.digits.as<mgcd_pair_t>(), offs, ks, sl.mr, outs, &sl.retired_dev,
+                                &sl.retired_host);
     if (crc != MGCD_OK)
       for (int& st : sts)
         if (st == MGCD_OK) st = crc;
