@@ -116,6 +116,27 @@ __device__ __forceinline__ void find_task(const MrArgs& a, uint32_t t, uint32_t&
   n = t - a.cum[lo];
 }
 
+// find_task for a whole wave that shares t: each step every lane probes
+// one of 64 evenly spaced cum entries and the ballot keeps the slice that
+// holds t, so 4,096 pairs take two dependent loads instead of twelve.
+// Probes are non-decreasing in the lane, hence so is cum[pos] <= t, and the
+// highest such lane L has cum[pos_L] <= t < cum[pos_(L+1)] (pos_(L+1) >
+// pos_L, or cum there would be <= t too).
+__device__ __forceinline__ void find_task_wave(const MrArgs& a, uint32_t t, uint32_t lane, uint32_t& p, uint32_t& n) {
+  uint32_t lo = 0, hi = a.npairs;  // cum[lo] <= t < cum[hi]
+  while (hi - lo > 1) {
+    const uint64_t span = hi - lo;
+    const uint32_t pos = lo + (uint32_t)(((uint64_t)lane * span) / kMrChunk);
+    const uint64_t mask = __ballot(a.cum[pos] <= t);  // lane 0 (pos = lo) always set
+    const uint32_t L = 63u - (uint32_t)__clzll((unsigned long long)mask);
+    const uint32_t nlo = lo + (uint32_t)(((uint64_t)L * span) / kMrChunk);
+    if (L + 1 < kMrChunk) hi = lo + (uint32_t)(((uint64_t)(L + 1) * span) / kMrChunk);
+    lo = nlo;
+  }
+  p = lo;
+  n = t - a.cum[lo];
+}
+
 // Level j -> j + 1, one task per thread (small nodes).
 __global__ void mr_level_small(MrArgs a) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -183,7 +204,7 @@ __global__ __launch_bounds__(kMrChunk) void mr_level_wave(MrArgs a, uint64_t uni
     const bool vpart = ((u / cpt) & 1) == 0;
     const uint64_t q = u % cpt;
     uint32_t p, n;
-    find_task(a, (uint32_t)task, p, n);
+    find_task_wave(a, (uint32_t)task, lane, p, n);
     const MrPair P = a.pairs[p];
     const Node lo = node_at(a, P, 2 * n), hi = node_at(a, P, 2 * n + 1);
     const uint64_t s = (uint64_t)lo.s + hi.s;
@@ -271,7 +292,7 @@ __global__ __launch_bounds__(kMrChunk) void mr_fix(MrArgs a, uint64_t pairs_of_u
     const uint64_t task = u2 / 2;
     const bool vpart = (u2 & 1) == 0;
     uint32_t p, n;
-    find_task(a, (uint32_t)task, p, n);
+    find_task_wave(a, (uint32_t)task, lane, p, n);
     const MrPair P = a.pairs[p];
     const Node lo = node_at(a, P, 2 * n), hi = node_at(a, P, 2 * n + 1);
     const uint64_t s = (uint64_t)lo.s + hi.s;
