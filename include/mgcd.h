@@ -113,7 +113,7 @@ enum {
    * parts). */
   MGCD_OPT_RELAY_PARTS = 7,
   /* 1 (default): the signed mixed-radix digits become positional limbs on
-   * the device, by a product tree (batches; one-pair calls above 64 digits),
+   * the device, by a product tree (batches; one-pair calls above 640 digits),
    * and only |G| is copied back.  0: the digits are copied back and
    * converted by the host Horner loop (GmpCudaDevice-gcd.cu:113-129). */
   MGCD_OPT_DEVICE_CONVERT = 8,
