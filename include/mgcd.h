@@ -158,9 +158,13 @@ int mgcd_ctx_last_pair_stats(const mgcd_ctx* ctx, uint32_t* nq, uint32_t* iters,
  * the per-pair bookkeeping, the barrier ending a round, the step from
  * elimination to replay, replay, recovery; out[9] elimination rounds;
  * out[10] work items; out[11] 100 MHz real-time ticks and out[12] shader
- * cycles over whole items (their ratio gives the clock).  Copies min(n, 16)
- * values; reset != 0 clears the sums afterwards.  No counterpart in the
- * reference. */
+ * cycles over whole items (their ratio gives the clock); out[13..15] for the
+ * one-pass collects (groups of up to 256 exchange slots): cycles from issuing
+ * a round's first slot loads to their return, the collects whose first loads
+ * found every slot posted, and the collects; out[16 + w] and out[24 + w]
+ * wave w's own elimination compute and its wait at the barrier after it
+ * (waves 0-7).  Copies min(n, 32) values; reset != 0 clears the sums
+ * afterwards.  No counterpart in the reference. */
 int mgcd_ctx_phase_clocks(mgcd_ctx* ctx, uint64_t* out, size_t n, int reset);
 
 /* One pair, one launch sequence: the path of GmpCudaDevice::gcd

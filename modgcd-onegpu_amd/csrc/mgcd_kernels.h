@@ -82,7 +82,7 @@ struct KernelArgs {
 
 // Phase sums of the instrumented build (-DMGCD_PHASE_CLOCKS, lib/libmgcd_phases.so):
 // see mgcd_ctx_phase_clocks.
-constexpr int kPhaseClocks = 16;
+constexpr int kPhaseClocks = 32;
 // True in the instrumented build.
 bool phase_clocks_built();
 

@@ -91,9 +91,12 @@ enum : int {
   // one-pass collects (groups of up to 256 slots): shader cycles from issuing
   // a round's slot loads to their return, the collects whose first loads
   // found every slot posted, and the collects
-  PH_FIRST_LOAD, PH_COLLECT_READY, PH_COLLECTS
+  PH_FIRST_LOAD, PH_COLLECT_READY, PH_COLLECTS,
+  // every wave's own compute and barrier laps, by wave (0-7): the two waves
+  // of a SIMD need not finish a round's compute together
+  PH_WAVE_COMPUTE = 16, PH_WAVE_BARRIER = 24
 };
-static_assert(PH_COLLECTS + 1 <= kPhaseClocks, "phase clock slots");
+static_assert(PH_WAVE_BARRIER + 8 <= kPhaseClocks, "phase clock slots");
 struct PhaseClock {
 #ifdef MGCD_PHASE_CLOCKS
   uint64_t acc[PH_ROUNDS] = {};
@@ -120,11 +123,18 @@ struct PhaseClock {
     atomicAdd(out + PH_MEMTIME, (unsigned long long)(n - t0));
     for (int p = 0; p < 3; ++p) atomicAdd(out + PH_FIRST_LOAD + p, (unsigned long long)extra[p]);
   }
+  __device__ void flush_waves(unsigned long long* out) {
+    const uint32_t w = threadIdx.x >> 6;
+    if (!out || (threadIdx.x & 63) != 0 || w >= 8) return;
+    atomicAdd(out + PH_WAVE_COMPUTE + w, (unsigned long long)acc[PH_COMPUTE]);
+    atomicAdd(out + PH_WAVE_BARRIER + w, (unsigned long long)acc[PH_BARRIER]);
+  }
 #else
   __device__ void start() {}
   __device__ void lap(int) {}
   __device__ void round() {}
   __device__ void flush(unsigned long long*, bool) {}
+  __device__ void flush_waves(unsigned long long*) {}
 #endif
 };
 
@@ -1542,6 +1552,7 @@ __device__ void process_group(const KernelArgs& args, const GroupDesc& Gd, Share
   clk.acc_extra(PH_COLLECTS, g.C.collects);
 #endif
   clk.flush(args.clocks, threadIdx.x == 0);
+  clk.flush_waves(args.clocks);
 }
 
 // Next work item for this workgroup (thread 0): queue 0, then the queue of

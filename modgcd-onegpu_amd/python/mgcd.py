@@ -278,13 +278,15 @@ class Context:
 
     def phase_clocks(self, reset=True):
         """Phase sums of the instrumented build (MGCD_LIB=...libmgcd_phases.so): dict of shader
-        cycles per phase plus rounds, items, realtime ticks (100 MHz) and memtime cycles."""
-        out = np.zeros(16, np.uint64)
-        check(lib().mgcd_ctx_phase_clocks(self.h, out.ctypes.data_as(P64), 16, 1 if reset else 0),
+        cycles per phase plus rounds, items, realtime ticks (100 MHz) and memtime cycles, the
+        one-pass collect counts, and each wave's (0-7) compute and barrier cycles."""
+        out = np.zeros(32, np.uint64)
+        check(lib().mgcd_ctx_phase_clocks(self.h, out.ctypes.data_as(P64), 32, 1 if reset else 0),
               "mgcd_ctx_phase_clocks")
         d = {name: int(out[i]) for i, name in enumerate(self.PHASES)}
         d.update(rounds=int(out[9]), items=int(out[10]), realtime_ticks=int(out[11]), memtime_cycles=int(out[12]),
-                 first_load_cycles=int(out[13]), collects_ready=int(out[14]), collects=int(out[15]))
+                 first_load_cycles=int(out[13]), collects_ready=int(out[14]), collects=int(out[15]),
+                 wave_compute=[int(x) for x in out[16:24]], wave_barrier=[int(x) for x in out[24:32]])
         return d
 
     def last_pair_stats(self):

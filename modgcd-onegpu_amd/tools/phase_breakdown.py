@@ -39,6 +39,11 @@ def summarize(c):
            "share": {n: round(c[n] / tot, 4) for n in names},
            "per_round_us": {n: round(us(c[n] / rounds), 3) for n in names if n.startswith("elim_") and n != "elim_tail"}}
     out["per_round_us"]["total"] = round(sum(out["per_round_us"].values()), 3)
+    waves = [i for i, x in enumerate(c.get("wave_compute", [])) if x]
+    if waves:
+        # every wave's own compute and barrier wait per round
+        out["per_wave_round_us"] = {"compute": [round(us(c["wave_compute"][i] / rounds), 3) for i in waves],
+                                    "barrier": [round(us(c["wave_barrier"][i] / rounds), 3) for i in waves]}
     if c.get("collects"):
         # one-pass collects: the slot loads' round trip, and how often they found every slot posted
         out["collect_first_load_us"] = round(us(c["first_load_cycles"] / c["collects"]), 3)
