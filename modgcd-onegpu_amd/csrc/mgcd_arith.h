@@ -155,14 +155,16 @@ struct Arith22 {
   }
 };
 
-// One Horner step of the residue reduction: (r * 2^32 + limb) mod m, signed
-// reduced in and out (modMP, getGcdKernel.cu:297-324).  t = r*2^32 is exact;
-// x = t + limb is rounded but only feeds the quotient estimate; t - q*m is
-// an integer below 2^34 and exact in the FMA.
+// One Horner step of the residue reduction: r * 2^32 + limb (mod m)
+// (modMP, getGcdKernel.cu:297-324), reduced lazily: the quotient comes from
+// t = r * 2^32 alone (exact), so t - q*m is an integer of magnitude at most
+// m/2 + 1, exact in the FMA, and adding the limb (< 2^32) stays exact.  In
+// and out |r| < m/2 + 2^32 + 1, whatever the step count (each step reduces t
+// afresh); the caller reduces once after the last step.  Five FP64
+// operations; reducing t + limb instead keeps r signed reduced for six.
 __device__ __forceinline__ double horner_step(double r, double limb, ModD M) {
   const double t = r * 4294967296.0;
-  const double x = t + limb;
-  const double q = __builtin_rint(x * M.minv);
+  const double q = __builtin_rint(t * M.minv);
   return __builtin_fma(-q, M.m, t) + limb;
 }
 

@@ -619,6 +619,11 @@ __device__ void residues(GPtr<const uint32_t> (&x)[NOP], const uint32_t (&n)[NOP
     if (rows == K) steps(std::true_type{});
     else steps(std::false_type{});
   }
+  // The Horner steps reduce lazily (|r| < m/2 + 2^32 + 1): once to signed reduced.
+#pragma unroll
+  for (int o = 0; o < NOP; ++o)
+#pragma unroll
+    for (int k = 0; k < K; ++k) r[o][k] = reduce(r[o][k], M[k]);
 }
 
 // Per-pair bookkeeping, one lane per pair in wave 0 (getGcdKernel.cu:350-352,

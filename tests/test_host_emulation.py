@@ -11,6 +11,11 @@ one ulp, in four ways.  Under it:
   inputs, Fibonacci-like worst cases and operands that round to the same
   float as the modulus (a floor-quotient form of this loop had a q = 0 fixed
   point there and hung on the GPU);
+* the residue Horner step (horner_step, reduced lazily) and the reduction
+  after the last step give the exact residue of random, all-ones and
+  single-limb operands for the smallest, largest and random moduli of the
+  L = 22, 27 and 32 tables, and every intermediate value stays inside the
+  step's bound;
 * the exchange-key decoders (csrc/mgcd_keys.h) never yield a modulus index
   outside the pair's N_Q, whatever payload a slot holds (0, ABORT, a
   recovery key read as an elimination key, random bits), and round-trip
@@ -39,7 +44,7 @@ def _build(tmp_path, name):
 @pytest.fixture(scope="module")
 def tools(tmp_path_factory):
     d = tmp_path_factory.mktemp("host")
-    return {n: _build(d, n) for n in ("keys_check", "modinv_check")}, d
+    return {n: _build(d, n) for n in ("keys_check", "modinv_check", "horner_check")}, d
 
 
 def test_key_decoders(tools):
@@ -103,3 +108,23 @@ def test_modinv_matches_pow(tools, mgcd, L, mode):
     for i in range(0, len(cases), 97):  # the 128-bit check above covers all; pow() a sample
         _, m, y = cases[i]
         assert int(got[i]) == pow(y, -1, m), (m, y)
+
+
+@pytest.mark.parametrize("L", [22, 27, 32])
+def test_residue_horner_steps(tools, mgcd, L):
+    exes, d = tools
+    mods = _moduli(L, mgcd)
+    rnd = random.Random(90 + L)
+    ms = [int(mods[0]), int(mods[-1])] + [int(mods[rnd.randrange(len(mods))]) for _ in range(30)]
+    recs = []
+    for m in ms:
+        for limbs in ([rnd.getrandbits(32) for _ in range(300)], [0xFFFFFFFF] * 200, [0] * 5 + [0xFFFFFFFF],
+                      [0xFFFFFFFF] + [0] * 50, [rnd.getrandbits(32)], [m - 1, m, m + 1, 0xFFFFFFFF, 1]):
+            recs += [m, len(limbs)] + limbs
+    path = d / f"horner_{L}.bin"
+    np.array(recs, dtype="<i8").tofile(path)
+    out = subprocess.run([exes["horner_check"], str(path)], check=True, capture_output=True, text=True,
+                         timeout=120).stdout.split()
+    assert out[0] == "checked" and int(out[1]) == len(ms) * 6
+    assert int(out[3]) == 0, f"L={L}: {out[3]} residues differ from the exact ones"
+    assert int(out[5]) == 0, f"L={L}: {out[5]} intermediate values outside the step's bound"
