@@ -155,17 +155,20 @@ struct Arith22 {
   }
 };
 
-// One Horner step of the residue reduction: r * 2^32 + limb (mod m)
-// (modMP, getGcdKernel.cu:297-324), reduced lazily: the quotient comes from
-// t = r * 2^32 alone (exact), so t - q*m is an integer of magnitude at most
-// m/2 + 1, exact in the FMA, and adding the limb (< 2^32) stays exact.  In
-// and out |r| < m/2 + 2^32 + 1, whatever the step count (each step reduces t
-// afresh); the caller reduces once after the last step.  Five FP64
-// operations; reducing t + limb instead keeps r signed reduced for six.
-__device__ __forceinline__ double horner_step(double r, double limb, ModD M) {
-  const double t = r * 4294967296.0;
+// One Horner step of the residue reduction, r * 2^32 + limb (mod m) (modMP,
+// getGcdKernel.cu:297-324), with the limb added one step late: the running
+// residue is s + L for the state s and the last limb L, and the step takes
+// s and ls = L * 2^32 (the tile holds limbs scaled so):
+//   t = s * 2^32 + ls = (s + L) * 2^32   exact: s + L is an integer < 2^34,
+//   q = rint(t * minv),  s' = t - q*m    an integer, exact in the FMA.
+// t < 2^65 puts fl(t * minv) within t/m * 2^-52 <= 2^13 / m of t / m, so
+// |s'| <= m/2 + 2^13 (signed reduced, as above) whatever the step count;
+// after the last step the caller adds the last limb and reduces once.  Four FP64 operations
+// (reducing r * 2^32 + limb to signed reduced every step takes six).
+__device__ __forceinline__ double horner_step(double s, double ls, ModD M) {
+  const double t = __builtin_fma(s, 4294967296.0, ls);
   const double q = __builtin_rint(t * M.minv);
-  return __builtin_fma(-q, M.m, t) + limb;
+  return __builtin_fma(-q, M.m, t);
 }
 
 }  // namespace dev

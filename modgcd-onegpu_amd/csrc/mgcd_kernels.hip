@@ -581,9 +581,11 @@ __device__ void residues(GPtr<const uint32_t> (&x)[NOP], const uint32_t (&n)[NOP
                          const ModD (&M)[K], int rows, double* tile) {
   constexpr int TS = TL / NOP;
   uint32_t nmax = 0;
+  double ls[NOP];  // the last limb of each operand, times 2^32, not yet added (horner_step)
 #pragma unroll
   for (int o = 0; o < NOP; ++o) {
     nmax = max(nmax, n[o]);
+    ls[o] = 0.0;
 #pragma unroll
     for (int k = 0; k < K; ++k) r[o][k] = 0.0;
   }
@@ -596,7 +598,7 @@ __device__ void residues(GPtr<const uint32_t> (&x)[NOP], const uint32_t (&n)[NOP
       const uint32_t i = nmax - 1 - (s0 + jj);
 #pragma unroll
       for (int q = 0; q < NOP; ++q)
-        if ((uint32_t)q == o) tile[q * TS + jj] = i < n[q] ? (double)x[q][i] : 0.0;
+        if ((uint32_t)q == o) tile[q * TS + jj] = i < n[q] ? (double)x[q][i] * 4294967296.0 : 0.0;
     }
     __syncthreads();
     // Unrolled by eight with the group tilings' larger tiles (the limbs'
@@ -609,21 +611,22 @@ __device__ void residues(GPtr<const uint32_t> (&x)[NOP], const uint32_t (&n)[NOP
       for (uint32_t j = 0; j < len; ++j) {
 #pragma unroll
         for (int o = 0; o < NOP; ++o) {
-          const double limb = tile[o * TS + j];
+          const double next = tile[o * TS + j];
 #pragma unroll
           for (int k = 0; k < K; ++k)
-            if (decltype(all_rows)::value || k < rows) r[o][k] = horner_step(r[o][k], limb, M[k]);
+            if (decltype(all_rows)::value || k < rows) r[o][k] = horner_step(r[o][k], ls[o], M[k]);
+          ls[o] = next;
         }
       }
     };
     if (rows == K) steps(std::true_type{});
     else steps(std::false_type{});
   }
-  // The Horner steps reduce lazily (|r| < m/2 + 2^32 + 1): once to signed reduced.
+  // The last limb, then once to signed reduced (|s + L| < m/2 + 2^13 + 2^32).
 #pragma unroll
   for (int o = 0; o < NOP; ++o)
 #pragma unroll
-    for (int k = 0; k < K; ++k) r[o][k] = reduce(r[o][k], M[k]);
+    for (int k = 0; k < K; ++k) r[o][k] = reduce(r[o][k] + ls[o] * 0x1p-32, M[k]);
 }
 
 // Per-pair bookkeeping, one lane per pair in wave 0 (getGcdKernel.cu:350-352,

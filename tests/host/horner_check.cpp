@@ -5,11 +5,12 @@
 //
 //   horner_check <in>
 // in: int64 records (m, n, limb[n-1], ..., limb[0]) little-endian, most
-// significant limb first.  For every record the chain r = 0, r =
-// horner_step(r, limb) over the limbs, then reduce and canon, is compared
-// with the residue computed in 128-bit integers; every intermediate r is
-// also checked against the step's bound |r| < m/2 + 2^32 + 1 and for being
-// an integer.  Prints "checked N wrong W bound B".
+// significant limb first.  For every record the chain s = 0, ls = 0, then
+// s = horner_step(s, ls), ls = limb * 2^32 over the limbs, then the last
+// limb added, reduce and canon, is compared with the residue computed in
+// 128-bit integers; every intermediate s is also checked against the step's
+// bound |s| <= m/2 + 2^13 and for being an integer.  Prints
+// "checked N wrong W bound B".
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -36,17 +37,18 @@ int main(int argc, char** argv) {
     ModD M;
     M.m = (double)m;
     M.minv = 1.0 / M.m;
-    const double lim = (double)m / 2 + 4294967296.0 + 1.0;
-    double r = 0.0;
+    const double lim = (double)m / 2 + 8192.0;
+    double s = 0.0, ls = 0.0;
     unsigned __int128 exact = 0;
     for (int64_t j = 0; j < n; ++j) {
       const uint32_t limb = (uint32_t)in[i + (size_t)j];
-      r = horner_step(r, (double)limb, M);
-      if (!(std::fabs(r) < lim) || r != std::rint(r)) ++bound;
+      s = horner_step(s, ls, M);
+      ls = (double)limb * 4294967296.0;
+      if (!(std::fabs(s) <= lim) || s != std::rint(s)) ++bound;
       exact = ((exact << 32) + limb) % (unsigned __int128)m;
     }
     i += (size_t)n;
-    const double c = canon(reduce(r, M), M);
+    const double c = canon(reduce(s + ls * 0x1p-32, M), M);
     ++checked;
     if (!(c >= 0.0 && c < (double)m) || (uint64_t)c != (uint64_t)exact) ++wrong;
   }

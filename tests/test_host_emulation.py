@@ -11,8 +11,9 @@ one ulp, in four ways.  Under it:
   inputs, Fibonacci-like worst cases and operands that round to the same
   float as the modulus (a floor-quotient form of this loop had a q = 0 fixed
   point there and hung on the GPU);
-* the residue Horner step (horner_step, reduced lazily) and the reduction
-  after the last step give the exact residue of random, all-ones and
+* the residue Horner step (horner_step, reduced lazily, each limb added one
+  step late) and the reduction after the last step give the exact residue
+  of random, all-ones and
   single-limb operands for the smallest, largest and random moduli of the
   L = 22, 27 and 32 tables, and every intermediate value stays inside the
   step's bound;
